@@ -1,0 +1,179 @@
+/*
+ * fbf.h -- C-ABI of the MI355X filtered brute-force top-k library (libfbf.so).
+ *
+ * The library replaces the brute-force branch of the reference's query router
+ * (hybrid_graph.cpp:200-228 and :399-419) and the exact filtered ground-truth
+ * scan (utils.h:80-158).  Plain pointers and sizes only; every entry point
+ * returns an fbf_status (0 = success).  All functions are safe to call from
+ * many host threads at once on distinct streams; calls on one dataset that
+ * share a stream are serialised by that stream.
+ *
+ * Data conventions (shared with the reference's file formats, utils.h:257-276
+ * and :550-566): base rows are float32 [n][d] in original id order with a
+ * float32 label (integral category id) and a float32 timestamp per row; a
+ * query is float32 [d] plus a meta [type, v, l, r] with type 0 = no filter,
+ * 1 = label == v, 2 = l <= timestamp <= r (inclusive), 3 = both.
+ *
+ * Result conventions: per query, k (id, squared-L2 distance) pairs in
+ * ascending (distance, id) order; ties go to the smaller original id; when
+ * fewer than k rows pass the filter the tail is padded with id FBF_PAD_ID and
+ * distance +inf.  Exact modes compute each distance in the canonical FP32
+ * order (t = x_i - q_i; acc = fma(t, t, acc) for i = 0..d-1).
+ */
+#ifndef FBF_H_
+#define FBF_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FBF_PAD_ID 0xFFFFFFFFu
+#define FBF_CODE_BYTES 112 /* reference SQ8 code stride (distance.hpp:271,305,332) */
+
+typedef enum fbf_status {
+    FBF_OK = 0,
+    FBF_ERR_INVALID_ARG = 1,   /* null pointer, bad size, k out of range, bad filter type */
+    FBF_ERR_UNSUPPORTED = 2,   /* dimension or mode not built into this library */
+    FBF_ERR_UNREGISTERED = 3,  /* codes pointer not inside a registered code array */
+    FBF_ERR_DEVICE = 4,        /* a HIP call failed; fbf_last_error() has the text */
+    FBF_ERR_NO_REPR = 5        /* the dataset was created without the representation a mode needs */
+} fbf_status;
+
+/* Distance / selection modes of fbf_search. */
+typedef enum fbf_mode {
+    FBF_MODE_EXACT_F32 = 0, /* plain exact FP32 scan (canonical order), bit-exact ids */
+    FBF_MODE_SQ8 = 1,       /* per-dim uint8 scan (SQ8Quantizer semantics), FP32 rerank of k' */
+    FBF_MODE_BF16 = 2,      /* BF16 MFMA q.x + FP32 norms, FP32 rerank of k' */
+    FBF_MODE_FP8 = 3,       /* FP8 e4m3 (per-vector scale) MFMA q.x + FP32 norms, FP32 rerank */
+    FBF_MODE_SQ8SYM = 4     /* symmetric int8 scan (router's bruteforce()), FP32 rerank of k' */
+} fbf_mode;
+
+/* Representations built by fbf_dataset_create (bit flags). */
+#define FBF_REPR_F32 0x1u     /* FP32 scan copies in both sorted layouts (exact mode) */
+#define FBF_REPR_SQ8 0x2u     /* per-dimension uint8 codes (mode SQ8) */
+#define FBF_REPR_BF16 0x4u    /* BF16 MFMA tiles (mode BF16) */
+#define FBF_REPR_FP8 0x8u     /* FP8 e4m3 MFMA tiles + per-row scale (mode FP8) */
+#define FBF_REPR_SQ8SYM 0x10u /* symmetric int8 codes, global absmax alpha (mode SQ8SYM) */
+#define FBF_REPR_ALL 0x1Fu
+
+typedef struct fbf_dataset fbf_dataset;
+
+/* Search tuning.  Zero fields take the library defaults (or the autotuned
+ * values recorded by fbf_autotune).  kprime is the over-fetch depth of the
+ * approximate modes (ignored by FBF_MODE_EXACT_F32); chunk_rows is the rows
+ * per work item; group_queries the queries sharing one streamed tile. */
+typedef struct fbf_search_opts {
+    uint32_t mode;        /* fbf_mode */
+    uint32_t kprime;      /* 0 -> 4k */
+    uint32_t chunk_rows;  /* 0 -> default */
+    uint32_t group_queries;
+    uint32_t persistent;  /* 1 -> one persistent work-queue launch for the scan */
+    uint32_t reserved[3];
+} fbf_search_opts;
+
+/* Create a device-resident dataset on the current HIP device from host
+ * arrays: vecs [n][d] float32 (original id order), labels[n], timestamps[n].
+ * Sorts the rows by (label, timestamp, id) and (timestamp, id) -- the layouts
+ * of SortDataset (hybrid_graph.cpp:505-524) -- and builds the representations
+ * named in repr_flags.  d must be 100 in this build. */
+fbf_status fbf_dataset_create(const float *vecs, const float *labels, const float *timestamps,
+                              uint64_t n, uint32_t d, uint32_t repr_flags, fbf_dataset **out);
+
+/* As fbf_dataset_create with vecs already in device memory (labels and
+ * timestamps stay on the host, they drive the planner). */
+fbf_status fbf_dataset_create_device(const float *vecs_dev, const float *labels,
+                                     const float *timestamps, uint64_t n, uint32_t d,
+                                     uint32_t repr_flags, fbf_dataset **out);
+
+fbf_status fbf_dataset_destroy(fbf_dataset *ds);
+
+/* Number of rows a query's filter admits (the router's e - s,
+ * hybrid_graph.cpp:176-199) and its selectivity (e - s) / n. */
+fbf_status fbf_filter_count(const fbf_dataset *ds, const float *meta, uint64_t *count,
+                            float *selectivity);
+
+/* Batched filtered top-k.  queries_dev: device float32 [nq][d]; metas: HOST
+ * float32 [nq][4]; out_ids_dev / out_dists_dev: device [nq][k].  stream is a
+ * hipStream_t (NULL = default stream).  k must be in [1, 512]. */
+fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, const float *metas,
+                      uint64_t nq, uint32_t k, const fbf_search_opts *opts,
+                      uint32_t *out_ids_dev, float *out_dists_dev, void *stream);
+
+/* Host-memory convenience form of fbf_search (copies in and out, waits). */
+fbf_status fbf_search_host(fbf_dataset *ds, const float *queries, const float *metas,
+                           uint64_t nq, uint32_t k, const fbf_search_opts *opts,
+                           uint32_t *out_ids, float *out_dists);
+
+/* Quantiser parameters of the dataset: alpha of the symmetric int8 codes
+ * (SQ8SymmetricQuantizer::alpha, sq8_sym_quant.hpp:37-44) and the per-dim
+ * min / (max - min) of the uint8 codes (SQ8Quantizer::mi/dif,
+ * sq8_quant.hpp:30-47).  mi and dif receive d floats each (may be NULL). */
+fbf_status fbf_dataset_quant_params(const fbf_dataset *ds, float *alpha, float *mi, float *dif);
+
+/* ---------------------------------------------------------------------
+ * Reference-shaped entry points (bruteforce.h:244-249 and :267-273).
+ *
+ * Register a host array of 112-byte symmetric int8 codes once; the library
+ * uploads it and later resolves any codes_in pointer inside [host_base,
+ * host_base + bytes) to the device copy.  Unregistered pointers are uploaded
+ * per call (correct, slow).
+ * --------------------------------------------------------------------- */
+fbf_status fbf_register_codes(const void *host_base, uint64_t bytes);
+fbf_status fbf_unregister_codes(const void *host_base);
+
+/* bruteforce (bruteforce.h:244-265): SQ8-symmetric scan of npoints codes,
+ * the k smallest int distances.  closest_points receives positions relative
+ * to codes_in in ascending (distance, position) order -- the same SET as the
+ * reference's heap whenever the k-th distance is not tied.  dim, alpha and
+ * nqueries are accepted for signature parity (the reference ignores them).
+ * k > npoints pads with -1. */
+fbf_status fbf_bruteforce(size_t dim, size_t k, int *closest_points, size_t npoints,
+                          const char *codes_in, float alpha, size_t nqueries,
+                          const int8_t *queries_in);
+
+/* bruteforce_subgraph (bruteforce.h:267-289): as fbf_bruteforce, plus
+ * distances[j] = int_distance * alpha * alpha. */
+fbf_status fbf_bruteforce_subgraph(size_t dim, size_t k, int *closest_points, float *distances,
+                                   size_t npoints, const char *codes_in, float alpha,
+                                   size_t nqueries, const int8_t *queries_in);
+
+/* Batched form of the router's brute-force branch over one registered code
+ * array: per query i the slice [starts[i], starts[i] + counts[i]) of 112-byte
+ * codes, query code qcodes[i*112 .. +112], top k positions (relative to the
+ * slice start) and int distances, ascending.  All arrays are host memory. */
+fbf_status fbf_bruteforce_batch(const char *codes_base, const uint64_t *starts,
+                                const uint64_t *counts, const int8_t *qcodes, uint64_t nq,
+                                uint32_t k, int32_t *out_pos, int32_t *out_dist);
+
+/* ---------------------------------------------------------------------
+ * Measurement
+ * --------------------------------------------------------------------- */
+
+/* When enabled, every kernel launch of this dataset is bracketed by HIP
+ * events on its stream; fbf_profile_collect waits for them and returns the
+ * launch count and summed milliseconds of the kernel named `kernel`
+ * ("scan", "merge", "rerank", "build", or "all"), then clears the log. */
+fbf_status fbf_profile_enable(fbf_dataset *ds, int enable);
+fbf_status fbf_profile_collect(fbf_dataset *ds, const char *kernel, uint64_t *launches,
+                               double *total_ms);
+
+/* Work counters of the last fbf_search on this dataset: P = sum over queries
+ * of the filtered candidate count, the rows actually streamed by the scan
+ * (including tile padding), and the number of work items. */
+fbf_status fbf_last_search_stats(const fbf_dataset *ds, uint64_t *pairs, uint64_t *pairs_computed,
+                                 uint64_t *work_items);
+
+/* Text of the last error on the calling thread ("" when none). */
+const char *fbf_last_error(void);
+
+/* Library build string (kernel set, target). */
+const char *fbf_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FBF_H_ */

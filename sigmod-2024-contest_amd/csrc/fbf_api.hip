@@ -1,0 +1,810 @@
+// fbf_api.hip -- C-ABI entry points, device-resident dataset, batch planner.
+//
+// The planner is the router's range step (hybrid_graph.cpp:168-199) made
+// batch-wide: each query's filter becomes a contiguous range of one of two
+// sorted layouts (SortDataset, hybrid_graph.cpp:505-524); queries are ordered
+// by (layout, start, end) as ReadSortedQuery orders them (utils.h:514-537),
+// grouped so that up to 16 queries share each streamed tile, and every group's
+// row span is cut into work items of chunk_rows rows.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/fbf.h"
+#include "fbf_internal.hpp"
+
+using namespace fbf;
+
+// ---------------------------------------------------------------- errors
+static thread_local std::string g_err;
+
+static fbf_status fail(fbf_status s, const std::string &msg) {
+    g_err = msg;
+    return s;
+}
+
+#define HIP_TRY(expr)                                                                    \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(FBF_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// ---------------------------------------------------------------- device buffers
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    hipError_t reserve(size_t b) {
+        if (b <= bytes) return hipSuccess;
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        size_t nb = std::max(b, bytes + bytes / 2);
+        hipError_t e = hipMalloc(&p, nb);
+        if (e == hipSuccess) bytes = nb;
+        return e;
+    }
+    void release() {
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+struct HostPinned {
+    void *p = nullptr;
+    size_t bytes = 0;
+    hipError_t reserve(size_t b) {
+        if (b <= bytes) return hipSuccess;
+        if (p) hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+        size_t nb = std::max(b, bytes + bytes / 2);
+        hipError_t e = hipHostMalloc(&p, nb, hipHostMallocDefault);
+        if (e == hipSuccess) bytes = nb;
+        return e;
+    }
+    void release() {
+        if (p) hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+struct LayoutDev {
+    uint32_t *ids = nullptr;   // [n_pad] position -> original id (pad: kPadId)
+    float *soa = nullptr;      // FP32 scan copy
+};
+
+struct ProfEvent {
+    int kind;  // 0 scan, 1 merge, 2 rerank(part of merge), 3 build
+    hipEvent_t a, b;
+};
+
+// Per-call scratch: the uploaded plan, the per-(item, query) candidate lists
+// and the pinned staging buffer the plan is copied through.
+struct Scratch {
+    DevBuf d_items, d_iq, d_slot_off, d_qslots, d_lists, d_counts, d_queue, d_q, d_out_ids, d_out_d;
+    HostPinned h_stage;
+    void release() {
+        for (DevBuf *b : {&d_items, &d_iq, &d_slot_off, &d_qslots, &d_lists, &d_counts, &d_queue,
+                          &d_q, &d_out_ids, &d_out_d})
+            b->release();
+        h_stage.release();
+    }
+};
+
+struct fbf_dataset {
+    int device = 0;
+    uint64_t n = 0, n_pad = 0;
+    uint32_t d = 0;
+    uint32_t repr = 0;
+    // host metadata for the planner
+    std::vector<int32_t> lab_b;
+    std::vector<float> ts_b, ts_t;
+    // device data
+    float *base_rows = nullptr;  // [n][d] original order
+    LayoutDev lay[2];
+    Scratch scr;
+    std::mutex mu;
+    // measurement
+    bool profile = false;
+    std::vector<ProfEvent> events;
+    uint64_t last_pairs = 0, last_pairs_computed = 0, last_items = 0;
+};
+
+// ---------------------------------------------------------------- helpers
+static inline int32_t label_int(float v) { return (int32_t)v; }  // utils.h:99,109
+
+struct QRange {
+    uint32_t layout, s, e, qi;
+};
+
+// Range of one query (hybrid_graph.cpp:176-199 semantics, integer labels,
+// inclusive timestamp bounds).  Returns false for an unknown filter type.
+static bool query_range(const fbf_dataset *ds, const float *meta, QRange &r) {
+    const float tf = meta[0];
+    const int type = (int)tf;
+    if (!(tf == (float)type) || type < 0 || type > 3) return false;
+    const int32_t v = label_int(meta[1]);
+    const float l = meta[2], rr = meta[3];
+    const uint64_t n = ds->n;
+    if (type == 0) {
+        r.layout = 1; r.s = 0; r.e = (uint32_t)n;
+        return true;
+    }
+    if (type == 2) {
+        r.layout = 1;
+        const float *t = ds->ts_t.data();
+        const uint64_t s = std::lower_bound(t, t + n, l) - t;
+        const uint64_t e = std::upper_bound(t, t + n, rr) - t;
+        r.s = (uint32_t)s;
+        r.e = (uint32_t)std::max(s, e);
+        return true;
+    }
+    r.layout = 0;
+    const int32_t *lab = ds->lab_b.data();
+    const float *ts = ds->ts_b.data();
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint64_t m = (lo + hi) / 2;
+        bool below = lab[m] < v || (type == 3 && lab[m] == v && ts[m] < l);
+        if (below) lo = m + 1; else hi = m;
+    }
+    const uint64_t s = lo;
+    lo = 0; hi = n;
+    while (lo < hi) {
+        uint64_t m = (lo + hi) / 2;
+        bool notabove = lab[m] < v || (lab[m] == v && (type == 1 || ts[m] <= rr));
+        if (notabove) lo = m + 1; else hi = m;
+    }
+    r.s = (uint32_t)s;
+    r.e = (uint32_t)std::max(s, lo);
+    return true;
+}
+
+static void record_begin(fbf_dataset *ds, int kind, hipStream_t st, ProfEvent &ev) {
+    if (!ds->profile) return;
+    ev.kind = kind;
+    hipEventCreate(&ev.a);
+    hipEventCreate(&ev.b);
+    hipEventRecord(ev.a, st);
+}
+static void record_end(fbf_dataset *ds, hipStream_t st, ProfEvent &ev) {
+    if (!ds->profile) return;
+    hipEventRecord(ev.b, st);
+    ds->events.push_back(ev);
+}
+
+// ---------------------------------------------------------------- dataset create
+static fbf_status dataset_build(fbf_dataset *ds, const float *labels, const float *timestamps,
+                                hipStream_t st) {
+    const uint64_t n = ds->n;
+    std::vector<int32_t> lab(n);
+    for (uint64_t i = 0; i < n; ++i) lab[i] = label_int(labels[i]);
+    std::vector<uint32_t> pb(n), pt(n);
+    std::iota(pb.begin(), pb.end(), 0u);
+    std::iota(pt.begin(), pt.end(), 0u);
+    // SortDataset comparators (hybrid_graph.cpp:14-27) with id as the final key.
+    std::sort(pb.begin(), pb.end(), [&](uint32_t a, uint32_t b) {
+        if (lab[a] != lab[b]) return lab[a] < lab[b];
+        if (timestamps[a] != timestamps[b]) return timestamps[a] < timestamps[b];
+        return a < b;
+    });
+    std::sort(pt.begin(), pt.end(), [&](uint32_t a, uint32_t b) {
+        if (timestamps[a] != timestamps[b]) return timestamps[a] < timestamps[b];
+        return a < b;
+    });
+    ds->lab_b.resize(n);
+    ds->ts_b.resize(n);
+    ds->ts_t.resize(n);
+    for (uint64_t i = 0; i < n; ++i) {
+        ds->lab_b[i] = lab[pb[i]];
+        ds->ts_b[i] = timestamps[pb[i]];
+        ds->ts_t[i] = timestamps[pt[i]];
+    }
+    const uint64_t n_pad = ds->n_pad;
+    std::vector<uint32_t> *perms[2] = {&pb, &pt};
+    for (int L = 0; L < 2; ++L) {
+        std::vector<uint32_t> &p = *perms[L];
+        p.resize(n_pad, kPadId);
+        HIP_TRY(hipMalloc(&ds->lay[L].ids, n_pad * sizeof(uint32_t)));
+        HIP_TRY(hipMemcpy(ds->lay[L].ids, p.data(), n_pad * sizeof(uint32_t), hipMemcpyHostToDevice));
+        if (ds->repr & FBF_REPR_F32) {
+            HIP_TRY(hipMalloc(&ds->lay[L].soa, n_pad * kDim * sizeof(float)));
+            ProfEvent ev;
+            record_begin(ds, 3, st, ev);
+            HIP_TRY(launch_build_soa(ds->base_rows, ds->lay[L].ids, n, ds->lay[L].soa, st));
+            record_end(ds, st, ev);
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    return FBF_OK;
+}
+
+static fbf_status dataset_create_common(const float *vecs, bool vecs_on_device, const float *labels,
+                                        const float *timestamps, uint64_t n, uint32_t d,
+                                        uint32_t repr, fbf_dataset **out) {
+    if (!out || !vecs || !labels || !timestamps) return fail(FBF_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    if (n == 0 || n >= 0xFFFFFFFFull - 64) return fail(FBF_ERR_INVALID_ARG, "n out of range");
+    if (d != (uint32_t)kDim) return fail(FBF_ERR_UNSUPPORTED, "this build supports d = 100 only");
+    if (repr == 0) repr = FBF_REPR_F32;
+    auto *ds = new fbf_dataset();
+    hipGetDevice(&ds->device);
+    ds->n = n;
+    ds->n_pad = (n + kRowTile - 1) / kRowTile * kRowTile;
+    ds->d = d;
+    ds->repr = repr;
+    hipError_t e = hipMalloc(&ds->base_rows, n * d * sizeof(float));
+    if (e != hipSuccess) {
+        delete ds;
+        return fail(FBF_ERR_DEVICE, std::string("hipMalloc base: ") + hipGetErrorString(e));
+    }
+    e = hipMemcpy(ds->base_rows, vecs, n * d * sizeof(float),
+                  vecs_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        fbf_dataset_destroy(ds);
+        return fail(FBF_ERR_DEVICE, std::string("upload base: ") + hipGetErrorString(e));
+    }
+    fbf_status s = dataset_build(ds, labels, timestamps, nullptr);
+    if (s != FBF_OK) {
+        std::string m = g_err;
+        fbf_dataset_destroy(ds);
+        return fail(s, m);
+    }
+    *out = ds;
+    return FBF_OK;
+}
+
+extern "C" fbf_status fbf_dataset_create(const float *vecs, const float *labels, const float *timestamps,
+                                         uint64_t n, uint32_t d, uint32_t repr_flags, fbf_dataset **out) {
+    return dataset_create_common(vecs, false, labels, timestamps, n, d, repr_flags, out);
+}
+
+extern "C" fbf_status fbf_dataset_create_device(const float *vecs_dev, const float *labels,
+                                                const float *timestamps, uint64_t n, uint32_t d,
+                                                uint32_t repr_flags, fbf_dataset **out) {
+    return dataset_create_common(vecs_dev, true, labels, timestamps, n, d, repr_flags, out);
+}
+
+extern "C" fbf_status fbf_dataset_destroy(fbf_dataset *ds) {
+    if (!ds) return FBF_OK;
+    hipDeviceSynchronize();
+    if (ds->base_rows) hipFree(ds->base_rows);
+    for (auto &L : ds->lay) {
+        if (L.ids) hipFree(L.ids);
+        if (L.soa) hipFree(L.soa);
+    }
+    ds->scr.release();
+    for (auto &ev : ds->events) {
+        hipEventDestroy(ev.a);
+        hipEventDestroy(ev.b);
+    }
+    delete ds;
+    return FBF_OK;
+}
+
+extern "C" fbf_status fbf_filter_count(const fbf_dataset *ds, const float *meta, uint64_t *count,
+                                       float *selectivity) {
+    if (!ds || !meta) return fail(FBF_ERR_INVALID_ARG, "null argument");
+    QRange r;
+    if (!query_range(ds, meta, r)) return fail(FBF_ERR_INVALID_ARG, "bad filter type");
+    if (count) *count = r.e - r.s;
+    if (selectivity) *selectivity = (float)(r.e - r.s) / (float)ds->n;
+    return FBF_OK;
+}
+
+// ---------------------------------------------------------------- planner
+struct Plan {
+    std::vector<WorkItem> items;
+    std::vector<ItemQuery> iq;
+    std::vector<uint64_t> slot_off;  // [nq + 1]
+    std::vector<uint64_t> q_slots;
+    uint64_t pairs = 0, pairs_computed = 0;
+};
+
+static fbf_status compute_ranges(const fbf_dataset *ds, const float *metas, uint64_t nq,
+                                 std::vector<QRange> &qr) {
+    qr.resize(nq);
+    for (uint64_t i = 0; i < nq; ++i) {
+        if (!query_range(ds, metas + 4 * i, qr[i]))
+            return fail(FBF_ERR_INVALID_ARG, "query " + std::to_string(i) + ": bad filter type");
+        qr[i].qi = (uint32_t)i;
+    }
+    return FBF_OK;
+}
+
+// Group queries and cut their spans into work items.  qr[i].qi must be i.
+static void make_plan(const std::vector<QRange> &qr, uint32_t chunk_rows, uint32_t group_max, Plan &P) {
+    const uint64_t nq = qr.size();
+    for (const QRange &r : qr) P.pairs += r.e - r.s;
+    std::vector<uint32_t> order;
+    order.reserve(nq);
+    for (uint64_t i = 0; i < nq; ++i)
+        if (qr[i].e > qr[i].s) order.push_back((uint32_t)i);
+    std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        const QRange &x = qr[a], &y = qr[b];
+        if (x.layout != y.layout) return x.layout < y.layout;
+        if (x.s != y.s) return x.s < y.s;
+        if (x.e != y.e) return x.e < y.e;
+        return a < b;
+    });
+
+    std::vector<uint64_t> per_q_slots(nq, 0);
+    std::vector<std::pair<uint32_t, uint64_t>> slot_owner;  // (query, slot)
+    uint64_t slot_next = 0;
+
+    auto emit_group = [&](const std::vector<uint32_t> &g) {
+        uint32_t us = UINT32_MAX, ue = 0;
+        for (uint32_t qi : g) {
+            us = std::min(us, qr[qi].s);
+            ue = std::max(ue, qr[qi].e);
+        }
+        const uint32_t b0 = us / kRowTile * kRowTile;
+        for (uint64_t cb = b0; cb < ue; cb += chunk_rows) {
+            const uint32_t ce = (uint32_t)std::min<uint64_t>(cb + chunk_rows, ue);
+            WorkItem W{};
+            W.row_begin = (uint32_t)cb;
+            W.row_end = ce;
+            W.q_begin = (uint32_t)P.iq.size();
+            W.layout = qr[g[0]].layout;
+            W.slot_begin = slot_next;
+            for (uint32_t qi : g) {
+                if (qr[qi].s < ce && qr[qi].e > cb) {
+                    P.iq.push_back(ItemQuery{qi, qr[qi].s, qr[qi].e, 0});
+                    slot_owner.emplace_back(qi, slot_next++);
+                    per_q_slots[qi]++;
+                }
+            }
+            W.q_count = (uint32_t)(P.iq.size() - W.q_begin);
+            if (W.q_count == 0) continue;
+            const uint64_t tiles = (ce - cb + kRowTile - 1) / kRowTile;
+            P.pairs_computed += tiles * kRowTile * W.q_count;
+            P.items.push_back(W);
+        }
+    };
+
+    std::vector<uint32_t> g;
+    uint32_t g_layout = 0, g_s = 0, g_e = 0, g_maxlen = 0;
+    for (uint32_t qi : order) {
+        const QRange &r = qr[qi];
+        bool fits = !g.empty() && g.size() < group_max && r.layout == g_layout;
+        if (fits) {
+            const uint32_t ns = std::min(g_s, r.s), ne = std::max(g_e, r.e);
+            const uint32_t ml = std::max(g_maxlen, r.e - r.s);
+            // The shared span may exceed the longest member's range by 25%.
+            fits = (uint64_t)(ne - ns) * 4 <= (uint64_t)ml * 5 + 4 * kRowTile;
+        }
+        if (!fits) {
+            if (!g.empty()) emit_group(g);
+            g.clear();
+            g_layout = r.layout;
+            g_s = r.s;
+            g_e = r.e;
+            g_maxlen = 0;
+        }
+        g.push_back(qi);
+        g_s = std::min(g_s, r.s);
+        g_e = std::max(g_e, r.e);
+        g_maxlen = std::max(g_maxlen, r.e - r.s);
+    }
+    if (!g.empty()) emit_group(g);
+
+    P.slot_off.assign(nq + 1, 0);
+    for (uint64_t i = 0; i < nq; ++i) P.slot_off[i + 1] = P.slot_off[i] + per_q_slots[i];
+    P.q_slots.assign(slot_next, 0);
+    std::vector<uint64_t> fill(P.slot_off.begin(), P.slot_off.end() - 1);
+    for (auto &so : slot_owner) P.q_slots[fill[so.first]++] = so.second;
+}
+
+// Reserve scratch for a plan with list capacity B and copy the plan to the
+// device through pinned memory (asynchronous on st).
+static fbf_status upload_plan(Scratch &S, const Plan &P, uint64_t nq, int B, hipStream_t st) {
+    const uint64_t n_slots = P.q_slots.size();
+    HIP_TRY(S.d_items.reserve(std::max<size_t>(1, P.items.size()) * sizeof(WorkItem)));
+    HIP_TRY(S.d_iq.reserve(std::max<size_t>(1, P.iq.size()) * sizeof(ItemQuery)));
+    HIP_TRY(S.d_slot_off.reserve((nq + 1) * sizeof(uint64_t)));
+    HIP_TRY(S.d_qslots.reserve(std::max<size_t>(1, n_slots) * sizeof(uint64_t)));
+    HIP_TRY(S.d_lists.reserve(std::max<size_t>(1, n_slots) * (size_t)B * sizeof(uint64_t)));
+    HIP_TRY(S.d_counts.reserve(std::max<size_t>(1, n_slots) * sizeof(uint32_t)));
+    HIP_TRY(S.d_queue.reserve(64));
+    const size_t b_items = P.items.size() * sizeof(WorkItem);
+    const size_t b_iq = P.iq.size() * sizeof(ItemQuery);
+    const size_t b_off = (nq + 1) * sizeof(uint64_t);
+    const size_t b_qs = n_slots * sizeof(uint64_t);
+    HIP_TRY(hipStreamSynchronize(st));  // the staging buffer may still feed a previous call
+    HIP_TRY(S.h_stage.reserve(b_items + b_iq + b_off + b_qs + 64));
+    char *h = (char *)S.h_stage.p;
+    std::memcpy(h, P.items.data(), b_items);
+    std::memcpy(h + b_items, P.iq.data(), b_iq);
+    std::memcpy(h + b_items + b_iq, P.slot_off.data(), b_off);
+    std::memcpy(h + b_items + b_iq + b_off, P.q_slots.data(), b_qs);
+    if (b_items) HIP_TRY(hipMemcpyAsync(S.d_items.p, h, b_items, hipMemcpyHostToDevice, st));
+    if (b_iq) HIP_TRY(hipMemcpyAsync(S.d_iq.p, h + b_items, b_iq, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(S.d_slot_off.p, h + b_items + b_iq, b_off, hipMemcpyHostToDevice, st));
+    if (b_qs) HIP_TRY(hipMemcpyAsync(S.d_qslots.p, h + b_items + b_iq + b_off, b_qs, hipMemcpyHostToDevice, st));
+    return FBF_OK;
+}
+
+// Range of items of one layout (items are sorted by layout).
+static void layout_run(const Plan &P, uint32_t L, uint64_t &i0, uint64_t &i1) {
+    i0 = 0;
+    while (i0 < P.items.size() && P.items[i0].layout != L) ++i0;
+    i1 = i0;
+    while (i1 < P.items.size() && P.items[i1].layout == L) ++i1;
+}
+
+static MergeArgs merge_args(const Scratch &S, int B, uint64_t nq, uint32_t k, uint32_t kprime) {
+    MergeArgs m{};
+    m.slot_off = S.d_slot_off.as<uint64_t>();
+    m.q_slots = S.d_qslots.as<uint64_t>();
+    m.lists = S.d_lists.as<uint64_t>();
+    m.counts = S.d_counts.as<uint32_t>();
+    m.list_cap = (uint32_t)B;
+    m.nq = nq;
+    m.k = k;
+    m.kprime = kprime;
+    return m;
+}
+
+// ---------------------------------------------------------------- search
+static uint32_t default_chunk(const fbf_dataset *ds) {
+    (void)ds;
+    return 256 * 1024;
+}
+
+extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, const float *metas,
+                                 uint64_t nq, uint32_t k, const fbf_search_opts *opts,
+                                 uint32_t *out_ids_dev, float *out_dists_dev, void *stream) {
+    if (!ds || !metas || (nq && (!queries_dev || !out_ids_dev || !out_dists_dev)))
+        return fail(FBF_ERR_INVALID_ARG, "null argument");
+    if (k == 0 || k > 512) return fail(FBF_ERR_INVALID_ARG, "k must be in [1, 512]");
+    if (nq == 0) return FBF_OK;
+    fbf_search_opts o{};
+    if (opts) o = *opts;
+    if (o.mode != FBF_MODE_EXACT_F32) return fail(FBF_ERR_UNSUPPORTED, "mode not available yet");
+    if (!(ds->repr & FBF_REPR_F32)) return fail(FBF_ERR_NO_REPR, "dataset has no FP32 scan copy");
+    const uint32_t chunk = o.chunk_rows ? (o.chunk_rows + kRowTile - 1) / kRowTile * kRowTile
+                                        : default_chunk(ds);
+    const uint32_t gmax = o.group_queries ? std::min<uint32_t>(o.group_queries, kMaxGroup) : kMaxGroup;
+    hipStream_t st = (hipStream_t)stream;
+
+    std::lock_guard<std::mutex> lock(ds->mu);
+    HIP_TRY(hipSetDevice(ds->device));
+    std::vector<QRange> qr;
+    fbf_status s = compute_ranges(ds, metas, nq, qr);
+    if (s != FBF_OK) return s;
+    Plan P;
+    make_plan(qr, chunk, gmax, P);
+    ds->last_pairs = P.pairs;
+    ds->last_pairs_computed = P.pairs_computed;
+    ds->last_items = P.items.size();
+
+    const uint32_t K = k;
+    const int B = list_capacity((int)K);
+    Scratch &S = ds->scr;
+    s = upload_plan(S, P, nq, B, st);
+    if (s != FBF_OK) return s;
+
+    // Scan.  Each work item writes every list slot it owns (count and keys),
+    // so no clearing is needed.
+    for (uint32_t L = 0; L < 2; ++L) {
+        uint64_t i0, i1;
+        layout_run(P, L, i0, i1);
+        if (i1 == i0) continue;
+        ScanArgs a{};
+        a.items = S.d_items.as<WorkItem>() + i0;
+        a.n_items = i1 - i0;
+        a.iq = S.d_iq.as<ItemQuery>();
+        a.queries = queries_dev;
+        a.soa = ds->lay[L].soa;
+        a.ids = ds->lay[L].ids;
+        a.lists = S.d_lists.as<uint64_t>();
+        a.counts = S.d_counts.as<uint32_t>();
+        a.K = K;
+        a.queue = nullptr;
+        if (o.persistent) {
+            HIP_TRY(hipMemsetAsync(S.d_queue.p, 0, sizeof(uint32_t), st));
+            a.queue = S.d_queue.as<uint32_t>();
+        }
+        ProfEvent ev;
+        record_begin(ds, 0, st, ev);
+        HIP_TRY(launch_scan_f32(a, B, st));
+        record_end(ds, st, ev);
+    }
+
+    MergeArgs m = merge_args(S, B, nq, k, K);
+    m.queries = queries_dev;
+    m.base_rows = ds->base_rows;
+    m.out_ids = out_ids_dev;
+    m.out_dists = out_dists_dev;
+    m.rerank = 0;
+    m.key_int = 0;
+    ProfEvent ev;
+    record_begin(ds, 1, st, ev);
+    HIP_TRY(launch_merge(m, st));
+    record_end(ds, st, ev);
+    return FBF_OK;
+}
+
+extern "C" fbf_status fbf_search_host(fbf_dataset *ds, const float *queries, const float *metas,
+                                      uint64_t nq, uint32_t k, const fbf_search_opts *opts,
+                                      uint32_t *out_ids, float *out_dists) {
+    if (!ds || !queries || !metas || !out_ids || !out_dists) return fail(FBF_ERR_INVALID_ARG, "null argument");
+    if (nq == 0) return FBF_OK;
+    HIP_TRY(hipSetDevice(ds->device));
+    float *dq = nullptr, *dd = nullptr;
+    uint32_t *di = nullptr;
+    HIP_TRY(hipMalloc(&dq, nq * ds->d * sizeof(float)));
+    hipError_t e1 = hipMalloc(&di, nq * (size_t)k * sizeof(uint32_t));
+    hipError_t e2 = hipMalloc(&dd, nq * (size_t)k * sizeof(float));
+    fbf_status s = FBF_OK;
+    if (e1 != hipSuccess || e2 != hipSuccess) s = fail(FBF_ERR_DEVICE, "hipMalloc outputs");
+    hipStream_t st = nullptr;
+    if (s == FBF_OK && hipStreamCreate(&st) != hipSuccess) s = fail(FBF_ERR_DEVICE, "hipStreamCreate");
+    if (s == FBF_OK && hipMemcpyAsync(dq, queries, nq * ds->d * sizeof(float), hipMemcpyHostToDevice, st) != hipSuccess)
+        s = fail(FBF_ERR_DEVICE, "upload queries");
+    if (s == FBF_OK) s = fbf_search(ds, dq, metas, nq, k, opts, di, dd, st);
+    if (s == FBF_OK && (hipMemcpyAsync(out_ids, di, nq * (size_t)k * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                        hipMemcpyAsync(out_dists, dd, nq * (size_t)k * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                        hipStreamSynchronize(st) != hipSuccess))
+        s = fail(FBF_ERR_DEVICE, "download results");
+    if (st) hipStreamDestroy(st);
+    hipFree(dq);
+    if (di) hipFree(di);
+    if (dd) hipFree(dd);
+    return s;
+}
+
+extern "C" fbf_status fbf_dataset_quant_params(const fbf_dataset *ds, float *alpha, float *mi, float *dif) {
+    (void)ds; (void)alpha; (void)mi; (void)dif;
+    return fail(FBF_ERR_NO_REPR, "quantisers not built");
+}
+
+// ---------------------------------------------------------------- measurement
+extern "C" fbf_status fbf_profile_enable(fbf_dataset *ds, int enable) {
+    if (!ds) return fail(FBF_ERR_INVALID_ARG, "null dataset");
+    ds->profile = enable != 0;
+    return FBF_OK;
+}
+
+extern "C" fbf_status fbf_profile_collect(fbf_dataset *ds, const char *kernel, uint64_t *launches,
+                                          double *total_ms) {
+    if (!ds || !kernel) return fail(FBF_ERR_INVALID_ARG, "null argument");
+    int want = -1;
+    if (!std::strcmp(kernel, "scan")) want = 0;
+    else if (!std::strcmp(kernel, "merge")) want = 1;
+    else if (!std::strcmp(kernel, "rerank")) want = 2;
+    else if (!std::strcmp(kernel, "build")) want = 3;
+    else if (std::strcmp(kernel, "all")) return fail(FBF_ERR_INVALID_ARG, "unknown kernel name");
+    uint64_t cnt = 0;
+    double ms = 0.0;
+    std::vector<ProfEvent> keep;
+    for (auto &ev : ds->events) {
+        if (want >= 0 && ev.kind != want) {
+            keep.push_back(ev);
+            continue;
+        }
+        HIP_TRY(hipEventSynchronize(ev.b));
+        float t = 0.f;
+        HIP_TRY(hipEventElapsedTime(&t, ev.a, ev.b));
+        ms += t;
+        ++cnt;
+        hipEventDestroy(ev.a);
+        hipEventDestroy(ev.b);
+    }
+    ds->events.swap(keep);
+    if (launches) *launches = cnt;
+    if (total_ms) *total_ms = ms;
+    return FBF_OK;
+}
+
+extern "C" fbf_status fbf_last_search_stats(const fbf_dataset *ds, uint64_t *pairs, uint64_t *pairs_computed,
+                                            uint64_t *work_items) {
+    if (!ds) return fail(FBF_ERR_INVALID_ARG, "null dataset");
+    if (pairs) *pairs = ds->last_pairs;
+    if (pairs_computed) *pairs_computed = ds->last_pairs_computed;
+    if (work_items) *work_items = ds->last_items;
+    return FBF_OK;
+}
+
+extern "C" const char *fbf_last_error(void) { return g_err.c_str(); }
+
+extern "C" const char *fbf_version(void) { return "fbf 0.1 gfx950: exact-f32 sq8sym"; }
+
+// ---------------------------------------------------------------- reference-shaped entry points
+//
+// Registry of host code arrays uploaded once (the router's codes_both /
+// codes_time / bucket codes, hybrid_graph.cpp:155-161,505-524).  A codes_in
+// pointer inside a registered range resolves to the device copy.
+struct CodeReg {
+    uint64_t bytes;
+    int device;
+    int8_t *dev;
+};
+static std::mutex g_reg_mu;
+static std::map<uintptr_t, CodeReg> g_reg;
+
+extern "C" fbf_status fbf_register_codes(const void *host_base, uint64_t bytes) {
+    if (!host_base || bytes == 0) return fail(FBF_ERR_INVALID_ARG, "null or empty code array");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    int8_t *d = nullptr;
+    HIP_TRY(hipMalloc(&d, bytes));
+    hipError_t e = hipMemcpy(d, host_base, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        return fail(FBF_ERR_DEVICE, std::string("upload codes: ") + hipGetErrorString(e));
+    }
+    std::lock_guard<std::mutex> lk(g_reg_mu);
+    auto it = g_reg.find((uintptr_t)host_base);
+    if (it != g_reg.end()) {
+        (void)hipFree(it->second.dev);
+        g_reg.erase(it);
+    }
+    g_reg[(uintptr_t)host_base] = CodeReg{bytes, dev, d};
+    return FBF_OK;
+}
+
+extern "C" fbf_status fbf_unregister_codes(const void *host_base) {
+    std::lock_guard<std::mutex> lk(g_reg_mu);
+    auto it = g_reg.find((uintptr_t)host_base);
+    if (it == g_reg.end()) return fail(FBF_ERR_UNREGISTERED, "pointer was not registered");
+    (void)hipFree(it->second.dev);
+    g_reg.erase(it);
+    return FBF_OK;
+}
+
+// Device address of [p, p + bytes) when it lies inside one registered array.
+static const int8_t *resolve_codes(const void *p, uint64_t bytes) {
+    std::lock_guard<std::mutex> lk(g_reg_mu);
+    const uintptr_t a = (uintptr_t)p;
+    auto it = g_reg.upper_bound(a);
+    if (it == g_reg.begin()) return nullptr;
+    --it;
+    const uintptr_t base = it->first;
+    if (a < base || a + bytes > base + it->second.bytes) return nullptr;
+    if ((a - base) % 16) return nullptr;  // 16-byte loads need 16-byte alignment
+    return it->second.dev + (a - base);
+}
+
+// Calling thread's engine for dataset-free calls: own stream and scratch, so
+// the router's OpenMP threads do not serialise on one another.
+struct ThreadEngine {
+    hipStream_t st = nullptr;
+    Scratch scr;
+    DevBuf tmp_codes;
+    HostPinned h_out;
+    ~ThreadEngine() {
+        if (st) (void)hipStreamDestroy(st);
+        scr.release();
+        tmp_codes.release();
+        h_out.release();
+    }
+};
+static thread_local ThreadEngine t_eng;
+
+// Symmetric int8 top-k over nq slices of one device code array.  out_pos /
+// out_dist are host arrays [nq][k]; positions are relative to each slice.
+static fbf_status sym_topk(const int8_t *codes_dev, uint64_t n_rows, const uint64_t *starts,
+                           const uint64_t *counts, const int8_t *qcodes_host, uint64_t nq, uint32_t k,
+                           int32_t *out_pos, int32_t *out_dist) {
+    ThreadEngine &E = t_eng;
+    if (!E.st) HIP_TRY(hipStreamCreateWithFlags(&E.st, hipStreamNonBlocking));
+    hipStream_t st = E.st;
+    std::vector<QRange> qr(nq);
+    for (uint64_t i = 0; i < nq; ++i) {
+        if (starts[i] + counts[i] > n_rows || starts[i] + counts[i] >= 0xFFFFFFFFull)
+            return fail(FBF_ERR_INVALID_ARG, "slice outside the code array");
+        qr[i] = QRange{0, (uint32_t)starts[i], (uint32_t)(starts[i] + counts[i]), (uint32_t)i};
+    }
+    Plan P;
+    make_plan(qr, 64 * 1024, kMaxGroup, P);
+    const int B = list_capacity((int)k);
+    Scratch &S = E.scr;
+    fbf_status s = upload_plan(S, P, nq, B, st);
+    if (s != FBF_OK) return s;
+    HIP_TRY(S.d_q.reserve(nq * 112));
+    HIP_TRY(S.d_out_ids.reserve(nq * (size_t)k * 4));
+    HIP_TRY(S.d_out_d.reserve(nq * (size_t)k * 4));
+    HIP_TRY(hipMemcpyAsync(S.d_q.p, qcodes_host, nq * 112, hipMemcpyHostToDevice, st));
+    ScanSymArgs a{};
+    a.items = S.d_items.as<WorkItem>();
+    a.n_items = P.items.size();
+    a.iq = S.d_iq.as<ItemQuery>();
+    a.qcodes = S.d_q.as<int8_t>();
+    a.codes = codes_dev;
+    a.ids = nullptr;
+    a.n_rows = n_rows;
+    a.lists = S.d_lists.as<uint64_t>();
+    a.counts = S.d_counts.as<uint32_t>();
+    a.K = k;
+    a.queue = nullptr;
+    HIP_TRY(launch_scan_sq8sym(a, B, st));
+    MergeArgs m = merge_args(S, B, nq, k, k);
+    m.out_ids = S.d_out_ids.as<uint32_t>();
+    m.out_dists = S.d_out_d.as<float>();
+    m.rerank = 0;
+    m.key_int = 1;
+    HIP_TRY(launch_merge(m, st));
+    HIP_TRY(E.h_out.reserve(nq * (size_t)k * 8));
+    uint32_t *hp = (uint32_t *)E.h_out.p;
+    HIP_TRY(hipMemcpyAsync(hp, S.d_out_ids.p, nq * (size_t)k * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(hp + nq * (size_t)k, S.d_out_d.p, nq * (size_t)k * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (uint64_t i = 0; i < nq * (uint64_t)k; ++i) {
+        const uint32_t id = hp[i];
+        // Slice-relative position; the row id in the key is absolute.
+        out_pos[i] = id == kPadId ? -1 : (int32_t)(id - (uint32_t)starts[i / k]);
+        out_dist[i] = id == kPadId ? INT32_MAX : (int32_t)hp[nq * (size_t)k + i];
+    }
+    return FBF_OK;
+}
+
+static fbf_status bruteforce_common(size_t k, int *closest_points, float *distances, size_t npoints,
+                                    const char *codes_in, float alpha, const int8_t *queries_in) {
+    if (k == 0) return FBF_OK;  // the reference writes nothing either
+    if (!closest_points || !queries_in || (npoints && !codes_in))
+        return fail(FBF_ERR_INVALID_ARG, "null argument");
+    if (k > 1024) return fail(FBF_ERR_INVALID_ARG, "k must be <= 1024");
+    std::vector<int32_t> pos(k, -1), dist(k, INT32_MAX);
+    if (npoints > 0) {
+        const uint64_t bytes = (uint64_t)npoints * 112;
+        const int8_t *dev = resolve_codes(codes_in, bytes);
+        if (!dev) {
+            // Not registered: stage this slice (correct, but a copy per call).
+            ThreadEngine &E = t_eng;
+            if (!E.st) HIP_TRY(hipStreamCreateWithFlags(&E.st, hipStreamNonBlocking));
+            HIP_TRY(E.tmp_codes.reserve(bytes));
+            HIP_TRY(hipMemcpyAsync(E.tmp_codes.p, codes_in, bytes, hipMemcpyHostToDevice, E.st));
+            dev = E.tmp_codes.as<int8_t>();
+        }
+        const uint64_t st0 = 0, cnt = npoints;
+        fbf_status s = sym_topk(dev, npoints, &st0, &cnt, queries_in, 1, (uint32_t)k, pos.data(), dist.data());
+        if (s != FBF_OK) return s;
+    }
+    for (size_t j = 0; j < k; ++j) {
+        closest_points[j] = pos[j];
+        if (distances) distances[j] = pos[j] < 0 ? HUGE_VALF : (float)dist[j] * alpha * alpha;
+    }
+    return FBF_OK;
+}
+
+extern "C" fbf_status fbf_bruteforce(size_t dim, size_t k, int *closest_points, size_t npoints,
+                                     const char *codes_in, float alpha, size_t nqueries,
+                                     const int8_t *queries_in) {
+    (void)dim; (void)nqueries;
+    return bruteforce_common(k, closest_points, nullptr, npoints, codes_in, alpha, queries_in);
+}
+
+extern "C" fbf_status fbf_bruteforce_subgraph(size_t dim, size_t k, int *closest_points, float *distances,
+                                              size_t npoints, const char *codes_in, float alpha,
+                                              size_t nqueries, const int8_t *queries_in) {
+    (void)dim; (void)nqueries;
+    if (!distances) return fail(FBF_ERR_INVALID_ARG, "null distances");
+    return bruteforce_common(k, closest_points, distances, npoints, codes_in, alpha, queries_in);
+}
+
+extern "C" fbf_status fbf_bruteforce_batch(const char *codes_base, const uint64_t *starts,
+                                           const uint64_t *counts, const int8_t *qcodes, uint64_t nq,
+                                           uint32_t k, int32_t *out_pos, int32_t *out_dist) {
+    if (!codes_base || !starts || !counts || !qcodes || !out_pos || !out_dist)
+        return fail(FBF_ERR_INVALID_ARG, "null argument");
+    if (k == 0 || k > 1024) return fail(FBF_ERR_INVALID_ARG, "k must be in [1, 1024]");
+    if (nq == 0) return FBF_OK;
+    uint64_t rows = 0;
+    for (uint64_t i = 0; i < nq; ++i) rows = std::max(rows, starts[i] + counts[i]);
+    const int8_t *dev = resolve_codes(codes_base, rows * 112);
+    if (!dev) return fail(FBF_ERR_UNREGISTERED, "codes_base is not inside a registered code array");
+    return sym_topk(dev, rows, starts, counts, qcodes, nq, k, out_pos, out_dist);
+}

@@ -1,0 +1,126 @@
+// fbf_internal.hpp -- types shared by the host planner and the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fbf {
+
+constexpr int kDim = 100;        // the reference hard-codes d = 100 (utils.h:255)
+constexpr int kRowTile = 64;     // rows per wave tile: one row per lane
+constexpr uint32_t kPadId = 0xFFFFFFFFu;
+constexpr uint64_t kKeyMax = ~0ull;
+
+// A work item: one wave scans rows [row_begin, row_end) of one sorted layout
+// for up to kMaxGroup queries.  Query j of the item is entry q_begin + j of
+// the plan's per-item query table; its candidate list lives in slot
+// slot_begin + j.
+struct WorkItem {
+    uint32_t row_begin;   // multiple of kRowTile
+    uint32_t row_end;
+    uint32_t q_begin;
+    uint32_t q_count;
+    uint32_t layout;      // 0 = (label, ts) order, 1 = ts order
+    uint32_t pad;
+    uint64_t slot_begin;
+};
+
+// Per (work item, query) entry: the query's index in the batch and its own
+// candidate range in the item's layout.
+struct ItemQuery {
+    uint32_t qi;
+    uint32_t s;
+    uint32_t e;
+    uint32_t pad;
+};
+
+constexpr int kMaxGroup = 16;    // queries per work item
+
+// (distance, id) key: distance bits in the high word.  For the non-negative
+// distances of the exact scan the float bits are already ordered; the
+// general map also orders negative values (approximate scores).
+__host__ __device__ inline uint32_t f2ord(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__host__ __device__ inline float ord2f(uint32_t o) {
+    uint32_t u = (o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o;
+    return __builtin_bit_cast(float, u);
+}
+
+// Read-only data that every lane of a wave reads at the same address (work
+// items, query vectors) is accessed through the constant address space so the
+// compiler issues scalar loads (s_load_*) into SGPRs; the buffers are never
+// written while a kernel that reads them runs.
+// (The host pass of hipcc parses device functions too; only the device pass
+// knows the constant address space, so the host pass sees plain pointers.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T> using cptr = const __attribute__((address_space(4))) T *;
+#else
+template <class T> using cptr = const T *;
+#endif
+template <class T> __device__ inline cptr<T> as_const(const T *p) { return (cptr<T>)p; }
+
+// List capacity for a kept depth K: room for K kept keys plus at least two
+// full tiles of appends.
+inline int list_capacity(int K) {
+    if (K <= 128) return 256;
+    if (K <= 384) return 512;
+    return 1024;
+}
+
+// ---------------------------------------------------------------- launchers
+struct ScanArgs {
+    const WorkItem *items;
+    uint64_t n_items;
+    const ItemQuery *iq;
+    const float *queries;      // [nq][kDim]
+    const float *soa;          // layout FP32 copy in 64-row x 4-dim blocks
+    const uint32_t *ids;       // layout position -> original id
+    uint64_t *lists;           // [slots][B]
+    uint32_t *counts;          // [slots]
+    uint32_t K;
+    uint32_t *queue;           // persistent mode work counter (or null)
+};
+
+// Symmetric int8 scan (bruteforce.h:244-289 over L2SqrSQ8_sym): rows are
+// 112-byte codes, keys are (int distance, id) with id = ids[row] when ids is
+// given, else the row position itself.
+struct ScanSymArgs {
+    const WorkItem *items;
+    uint64_t n_items;
+    const ItemQuery *iq;
+    const int8_t *qcodes;      // [nq][112]
+    const int8_t *codes;       // [rows][112]
+    const uint32_t *ids;       // may be null
+    uint64_t n_rows;           // rows readable from codes (tiles are clamped to it)
+    uint64_t *lists;
+    uint32_t *counts;
+    uint32_t K;
+    uint32_t *queue;
+};
+
+struct MergeArgs {
+    const uint64_t *slot_off;  // [nq + 1] CSR over q_slots
+    const uint64_t *q_slots;   // slot indices
+    const uint64_t *lists;
+    const uint32_t *counts;
+    uint32_t list_cap;         // B of the scan lists
+    uint64_t nq;
+    uint32_t k;                // outputs per query
+    uint32_t kprime;           // merged depth before rerank (== k for exact)
+    const float *queries;      // rerank only
+    const float *base_rows;    // rerank only: [n][kDim] original order
+    uint32_t *out_ids;
+    float *out_dists;          // float distances (key_int == 0) or raw uint32 (key_int == 1)
+    int rerank;
+    int key_int;               // keys carry an integer distance (symmetric int8 codes)
+};
+
+hipError_t launch_scan_f32(const ScanArgs &a, int B, hipStream_t st);
+hipError_t launch_merge(const MergeArgs &a, hipStream_t st);
+hipError_t launch_scan_sq8sym(const ScanSymArgs &a, int B, hipStream_t st);
+hipError_t launch_build_soa(const float *rows, const uint32_t *perm, uint64_t n,
+                            float *soa, hipStream_t st);
+
+}  // namespace fbf

@@ -1,0 +1,258 @@
+// fbf_scan.hip -- exact FP32 filtered scan, merge / rerank, layout builders.
+//
+// Exact scan (replaces the scalar loop of Bruteforce, utils.h:105-139, and the
+// per-row normal_l2 / L2SqrRef calls, distance.h:62-71, distance.hpp:165-173):
+// one wave64 per work item, one candidate row per lane held in 100 VGPRs, the
+// item's queries read as wave-uniform scalars, so every (query, row) pair costs
+// one v_sub_f32 and one v_fma_f32 per dimension in the canonical order
+//     t = x_i - q_i;  acc = fma(t, t, acc),   i = 0 .. 99.
+// Rows stream from HBM once per item and are reused across the item's queries.
+#include "fbf_select.hpp"
+
+#pragma clang fp contract(off)
+
+namespace fbf {
+
+// ---------------------------------------------------------------- layout build
+
+// FP32 scan copy of one sorted layout: position p lives in block p / 64 as
+// 25 float4 groups, group g of lane p % 64 at float4 index (blk*25 + g)*64 + lane,
+// so a wave's load of group g is one contiguous 1 KiB burst.  Rows at or
+// beyond n (tile padding) are zero.
+__global__ void build_soa_kernel(const float *__restrict__ rows, const uint32_t *__restrict__ perm,
+                                 uint64_t n, uint64_t n_pad, float4 *__restrict__ soa) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t total = n_pad * (kDim / 4);
+    if (t >= total) return;
+    const uint64_t p = t % n_pad;        // consecutive threads take consecutive rows
+    const uint32_t g = (uint32_t)(t / n_pad);
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (p < n) {
+        const float *src = rows + (uint64_t)perm[p] * kDim + 4 * g;
+        v = make_float4(src[0], src[1], src[2], src[3]);
+    }
+    const uint64_t blk = p / kRowTile, lane = p % kRowTile;
+    soa[(blk * (kDim / 4) + g) * kRowTile + lane] = v;
+}
+
+hipError_t launch_build_soa(const float *rows, const uint32_t *perm, uint64_t n, float *soa,
+                            hipStream_t st) {
+    const uint64_t n_pad = (n + kRowTile - 1) / kRowTile * kRowTile;
+    const uint64_t total = n_pad * (kDim / 4);
+    const uint64_t blocks = (total + 255) / 256;
+    hipLaunchKernelGGL(build_soa_kernel, dim3((uint32_t)blocks), dim3(256), 0, st, rows, perm, n,
+                       n_pad, reinterpret_cast<float4 *>(soa));
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- exact scan
+
+
+__device__ inline void load_row(const float4 *__restrict__ soa, uint32_t tile_row0, uint32_t lane,
+                                float (&x)[kDim]) {
+    const float4 *src = soa + (uint64_t)(tile_row0 / kRowTile) * (kDim / 4) * kRowTile + lane;
+#pragma unroll
+    for (int g = 0; g < kDim / 4; ++g) {
+        const float4 v = src[g * kRowTile];
+        x[4 * g + 0] = v.x;
+        x[4 * g + 1] = v.y;
+        x[4 * g + 2] = v.z;
+        x[4 * g + 3] = v.w;
+    }
+}
+
+template <int B>
+__device__ void scan_f32_item(const ScanArgs &a, const WorkItem &W, uint64_t *sbuf,
+                              uint64_t *s_tau, uint32_t *s_cnt, float4 *sq) {
+    const uint32_t lane = threadIdx.x;
+    const uint32_t nq = W.q_count;
+    for (uint32_t j = 0; j < nq; ++j) {
+        s_tau[j] = kKeyMax;  // every lane writes the same value and reads only its own writes
+        s_cnt[j] = 0;
+    }
+    const float4 *soa = reinterpret_cast<const float4 *>(a.soa);
+    cptr<ItemQuery> iq = as_const(a.iq) + W.q_begin;
+    // The item's query vectors, staged once in LDS; the scan reads them at
+    // wave-uniform addresses (LDS broadcast).
+    __syncthreads();
+    for (uint32_t e = lane; e < nq * (kDim / 4); e += 64) {
+        const uint32_t j = e / (kDim / 4), g = e % (kDim / 4);
+        const float4 *src = reinterpret_cast<const float4 *>(a.queries + (uint64_t)iq[j].qi * kDim);
+        sq[j * (kDim / 4) + g] = src[g];
+    }
+    __syncthreads();
+
+#pragma unroll 1
+    for (uint32_t t = W.row_begin; t < W.row_end; t += kRowTile) {
+        const uint32_t row = t + lane;
+        float x[kDim];
+        load_row(soa, t, lane, x);
+        const uint32_t id = a.ids[row];  // ids are padded to the tile multiple
+        const bool row_ok = row < W.row_end;
+
+#pragma unroll 1
+        for (uint32_t j = 0; j < nq; ++j) {
+            const ItemQuery e = iq[j];
+            if (t >= e.e || t + kRowTile <= e.s) continue;  // tile outside this query's range
+            // One query at a time: the compiler keeps a single FMA chain and
+            // consumes each LDS broadcast read right away.
+            const float4 *qv = sq + j * (kDim / 4);
+            float acc = 0.f;
+#pragma unroll
+            for (int g = 0; g < kDim / 4; ++g) {
+                const float4 v = qv[g];
+                float tt;
+                tt = x[4 * g + 0] - v.x; acc = __builtin_fmaf(tt, tt, acc);
+                tt = x[4 * g + 1] - v.y; acc = __builtin_fmaf(tt, tt, acc);
+                tt = x[4 * g + 2] - v.z; acc = __builtin_fmaf(tt, tt, acc);
+                tt = x[4 * g + 3] - v.w; acc = __builtin_fmaf(tt, tt, acc);
+            }
+            const bool in = row_ok && row >= e.s && row < e.e;
+            const uint64_t key = ((uint64_t)f2ord(acc) << 32) | id;
+            uint64_t tau = s_tau[j];
+            uint32_t cnt = s_cnt[j];
+            uint64_t *list = a.lists + (W.slot_begin + j) * (uint64_t)B;
+            offer_global<B>(list, cnt, tau, a.K, sbuf, in && key < tau, key);
+            s_tau[j] = tau;
+            s_cnt[j] = cnt;
+        }
+    }
+    // Final cut: each list sorted and at most K long.
+    for (uint32_t j = 0; j < nq; ++j) {
+        uint64_t tau = s_tau[j];
+        uint32_t cnt = s_cnt[j];
+        uint64_t *list = a.lists + (W.slot_begin + j) * (uint64_t)B;
+        compact_global<B>(list, cnt, tau, a.K, sbuf);
+        if (lane == 0) a.counts[W.slot_begin + j] = cnt;
+    }
+}
+
+template <int B>
+__global__ __launch_bounds__(64) void scan_f32_kernel(ScanArgs a) {
+    __shared__ uint64_t sbuf[B];
+    __shared__ uint64_t s_tau[kMaxGroup];
+    __shared__ uint32_t s_cnt[kMaxGroup];
+    __shared__ float4 sq[kMaxGroup * (kDim / 4)];
+    if (a.queue == nullptr) {
+        const WorkItem W = as_const(a.items)[blockIdx.x];
+        scan_f32_item<B>(a, W, sbuf, s_tau, s_cnt, sq);
+        return;
+    }
+    // Persistent form: each wave pulls item indices until the queue is drained.
+    for (;;) {
+        uint32_t idx = 0;
+        if (threadIdx.x == 0) idx = atomicAdd(a.queue, 1u);
+        idx = __builtin_amdgcn_readfirstlane(__shfl(idx, 0));
+        if (idx >= a.n_items) break;
+        const WorkItem W = as_const(a.items)[idx];
+        scan_f32_item<B>(a, W, sbuf, s_tau, s_cnt, sq);
+    }
+}
+
+hipError_t launch_scan_f32(const ScanArgs &a, int B, hipStream_t st) {
+    if (a.n_items == 0) return hipSuccess;
+    uint64_t grid = a.n_items;
+    if (a.queue) {
+        int dev = 0, cus = 0;
+        hipGetDevice(&dev);
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        grid = (uint64_t)cus * 8;
+        if (grid > a.n_items) grid = a.n_items;
+    }
+    switch (B) {
+    case 256: hipLaunchKernelGGL(scan_f32_kernel<256>, dim3((uint32_t)grid), dim3(64), 0, st, a); break;
+    case 512: hipLaunchKernelGGL(scan_f32_kernel<512>, dim3((uint32_t)grid), dim3(64), 0, st, a); break;
+    case 1024: hipLaunchKernelGGL(scan_f32_kernel<1024>, dim3((uint32_t)grid), dim3(64), 0, st, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- merge / rerank
+
+// Canonical FP32 distance of a row-major original-order row (rerank gather).
+__device__ inline float canon_dist_row(const float *__restrict__ row, const float *__restrict__ q) {
+    const float4 *r4 = reinterpret_cast<const float4 *>(row);
+    float acc = 0.f;
+#pragma unroll
+    for (int g = 0; g < kDim / 4; ++g) {
+        const float4 v = r4[g];
+        float t;
+        t = v.x - q[4 * g + 0]; acc = __builtin_fmaf(t, t, acc);
+        t = v.y - q[4 * g + 1]; acc = __builtin_fmaf(t, t, acc);
+        t = v.z - q[4 * g + 2]; acc = __builtin_fmaf(t, t, acc);
+        t = v.w - q[4 * g + 3]; acc = __builtin_fmaf(t, t, acc);
+    }
+    return acc;
+}
+
+// One wave per query: stream the query's per-item lists through a running
+// LDS list, keep the kprime smallest keys; exact modes emit them, the
+// approximate modes recompute each candidate's canonical FP32 distance and
+// emit the k smallest (distance, id).
+template <int B>
+__global__ __launch_bounds__(64) void merge_kernel(MergeArgs a) {
+    __shared__ uint64_t list[B];
+    const uint64_t q = blockIdx.x;
+    const uint32_t lane = threadIdx.x;
+    uint32_t cnt = 0;
+    uint64_t tau = kKeyMax;
+    const uint64_t s0 = a.slot_off[q], s1 = a.slot_off[q + 1];
+#pragma unroll 1
+    for (uint64_t si = s0; si < s1; ++si) {
+        const uint64_t slot = a.q_slots[si];
+        const uint32_t c = a.counts[slot];
+        const uint64_t *L = a.lists + slot * (uint64_t)a.list_cap;
+#pragma unroll 1
+        for (uint32_t i = 0; i < c; i += 64) {
+            const uint32_t ii = i + lane;
+            const uint64_t key = ii < c ? L[ii] : kKeyMax;
+            offer_lds<B>(list, cnt, tau, a.kprime, key < tau, key);
+        }
+    }
+    compact_lds<B>(list, cnt, tau, a.kprime);  // sorted, cnt <= kprime
+
+    if (a.rerank) {
+        const float *__restrict__ qv = a.queries + q * kDim;
+        for (uint32_t i = lane; i < (uint32_t)B; i += 64) {
+            uint64_t nk = kKeyMax;
+            if (i < cnt) {
+                const uint32_t id = (uint32_t)list[i];
+                const float d = canon_dist_row(a.base_rows + (uint64_t)id * kDim, qv);
+                nk = ((uint64_t)f2ord(d) << 32) | id;
+            }
+            list[i] = nk;
+        }
+        __syncthreads();
+        wave_sort_lds<B>(list);
+        if (cnt > a.k) cnt = a.k;
+    }
+    for (uint32_t i = lane; i < a.k; i += 64) {
+        uint32_t id = kPadId;
+        float d = __builtin_inff();
+        if (i < cnt) {
+            const uint64_t key = list[i];
+            id = (uint32_t)key;
+            d = a.key_int ? __builtin_bit_cast(float, (uint32_t)(key >> 32)) : ord2f((uint32_t)(key >> 32));
+        } else if (a.key_int) {
+            d = __builtin_bit_cast(float, 0xFFFFFFFFu);
+        }
+        a.out_ids[q * a.k + i] = id;
+        a.out_dists[q * a.k + i] = d;
+    }
+}
+
+hipError_t launch_merge(const MergeArgs &a, hipStream_t st) {
+    if (a.nq == 0) return hipSuccess;
+    const uint32_t need = a.kprime > a.k ? a.kprime : a.k;
+    const int B = list_capacity((int)need);
+    switch (B) {
+    case 256: hipLaunchKernelGGL(merge_kernel<256>, dim3((uint32_t)a.nq), dim3(64), 0, st, a); break;
+    case 512: hipLaunchKernelGGL(merge_kernel<512>, dim3((uint32_t)a.nq), dim3(64), 0, st, a); break;
+    case 1024: hipLaunchKernelGGL(merge_kernel<1024>, dim3((uint32_t)a.nq), dim3(64), 0, st, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace fbf

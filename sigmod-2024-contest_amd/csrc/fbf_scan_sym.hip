@@ -1,0 +1,148 @@
+// fbf_scan_sym.hip -- symmetric int8 scan (the router's brute-force kernel).
+//
+// Replaces bruteforce() / bruteforce_subgraph() (bruteforce.h:244-289) and
+// their per-row L2SqrSQ8_sym (pyglass/glass/simd/distance.hpp:329-366): exact
+// int32 sum over the 112-byte code of (x_i - q_i)^2, evaluated as
+//     |x|^2 - 2 x.q + |q|^2
+// with v_dot4_i32_i8 (four int8 products per instruction, int32 accumulate;
+// every term is exact, so the result equals the reference bit for bit).
+// One wave per work item; a 64-row tile (7 KiB) is loaded with coalesced
+// 16-byte loads into LDS, then each lane owns one row.
+#include "fbf_select.hpp"
+
+namespace fbf {
+
+constexpr int kCodeInt4 = 7;  // 112 bytes = 7 x 16
+
+__device__ inline int dot_codes(const int4 (&x)[kCodeInt4], const int4 *q) {
+    int acc = 0;
+#pragma unroll
+    for (int c = 0; c < kCodeInt4; ++c) {
+        const int4 v = q[c];
+        acc = __builtin_amdgcn_sdot4(x[c].x, v.x, acc, false);
+        acc = __builtin_amdgcn_sdot4(x[c].y, v.y, acc, false);
+        acc = __builtin_amdgcn_sdot4(x[c].z, v.z, acc, false);
+        acc = __builtin_amdgcn_sdot4(x[c].w, v.w, acc, false);
+    }
+    return acc;
+}
+
+template <int B>
+__device__ void scan_sym_item(const ScanSymArgs &a, const WorkItem &W, uint64_t *sbuf,
+                              uint64_t *s_tau, uint32_t *s_cnt, int *s_qq, int4 *sq, int4 *stile) {
+    const uint32_t lane = threadIdx.x;
+    const uint32_t nq = W.q_count;
+    cptr<ItemQuery> iq = as_const(a.iq) + W.q_begin;
+    for (uint32_t j = 0; j < nq; ++j) {
+        s_tau[j] = kKeyMax;
+        s_cnt[j] = 0;
+    }
+    __syncthreads();
+    for (uint32_t e = lane; e < nq * kCodeInt4; e += 64) {
+        const uint32_t j = e / kCodeInt4, c = e % kCodeInt4;
+        sq[e] = reinterpret_cast<const int4 *>(a.qcodes + (uint64_t)iq[j].qi * 112)[c];
+    }
+    __syncthreads();
+    if (lane < nq) {
+        int4 qx[kCodeInt4];
+#pragma unroll
+        for (int c = 0; c < kCodeInt4; ++c) qx[c] = sq[lane * kCodeInt4 + c];
+        s_qq[lane] = dot_codes(qx, sq + lane * kCodeInt4);
+    }
+    __syncthreads();
+
+    const int4 *src = reinterpret_cast<const int4 *>(a.codes);
+#pragma unroll 1
+    for (uint32_t t = W.row_begin; t < W.row_end; t += kRowTile) {
+        // Coalesced tile load: int4 number (t*7 + lane + 64c) of the code array.
+#pragma unroll
+        for (int c = 0; c < kCodeInt4; ++c) {
+            const uint32_t e = lane + 64 * c;
+            const uint64_t row = t + e / kCodeInt4;
+            int4 v = make_int4(0, 0, 0, 0);
+            if (row < a.n_rows) v = src[(uint64_t)t * kCodeInt4 + e];
+            stile[e] = v;
+        }
+        __syncthreads();
+        int4 x[kCodeInt4];
+#pragma unroll
+        for (int c = 0; c < kCodeInt4; ++c) x[c] = stile[lane * kCodeInt4 + c];
+        __syncthreads();
+        const uint32_t row = t + lane;
+        int xxs = 0;
+#pragma unroll
+        for (int c = 0; c < kCodeInt4; ++c) {
+            xxs = __builtin_amdgcn_sdot4(x[c].x, x[c].x, xxs, false);
+            xxs = __builtin_amdgcn_sdot4(x[c].y, x[c].y, xxs, false);
+            xxs = __builtin_amdgcn_sdot4(x[c].z, x[c].z, xxs, false);
+            xxs = __builtin_amdgcn_sdot4(x[c].w, x[c].w, xxs, false);
+        }
+        const bool row_ok = row < W.row_end;
+        const uint32_t id = (a.ids && row_ok) ? a.ids[row] : row;
+#pragma unroll 1
+        for (uint32_t j = 0; j < nq; ++j) {
+            const ItemQuery e = iq[j];
+            if (t >= e.e || t + kRowTile <= e.s) continue;
+            const int xq = dot_codes(x, sq + j * kCodeInt4);
+            const int dist = xxs - 2 * xq + s_qq[j];
+            const bool in = row_ok && row >= e.s && row < e.e;
+            const uint64_t key = ((uint64_t)(uint32_t)dist << 32) | id;
+            uint64_t tau = s_tau[j];
+            uint32_t cnt = s_cnt[j];
+            uint64_t *list = a.lists + (W.slot_begin + j) * (uint64_t)B;
+            offer_global<B>(list, cnt, tau, a.K, sbuf, in && key < tau, key);
+            s_tau[j] = tau;
+            s_cnt[j] = cnt;
+        }
+    }
+    for (uint32_t j = 0; j < nq; ++j) {
+        uint64_t tau = s_tau[j];
+        uint32_t cnt = s_cnt[j];
+        uint64_t *list = a.lists + (W.slot_begin + j) * (uint64_t)B;
+        compact_global<B>(list, cnt, tau, a.K, sbuf);
+        if (lane == 0) a.counts[W.slot_begin + j] = cnt;
+    }
+}
+
+template <int B>
+__global__ __launch_bounds__(64) void scan_sym_kernel(ScanSymArgs a) {
+    __shared__ uint64_t sbuf[B];
+    __shared__ uint64_t s_tau[kMaxGroup];
+    __shared__ uint32_t s_cnt[kMaxGroup];
+    __shared__ int s_qq[kMaxGroup];
+    __shared__ int4 sq[kMaxGroup * kCodeInt4];
+    __shared__ int4 stile[kRowTile * kCodeInt4];
+    if (a.queue == nullptr) {
+        const WorkItem W = as_const(a.items)[blockIdx.x];
+        scan_sym_item<B>(a, W, sbuf, s_tau, s_cnt, s_qq, sq, stile);
+        return;
+    }
+    for (;;) {
+        uint32_t idx = 0;
+        if (threadIdx.x == 0) idx = atomicAdd(a.queue, 1u);
+        idx = __builtin_amdgcn_readfirstlane(__shfl(idx, 0));
+        if (idx >= a.n_items) break;
+        const WorkItem W = as_const(a.items)[idx];
+        scan_sym_item<B>(a, W, sbuf, s_tau, s_cnt, s_qq, sq, stile);
+    }
+}
+
+hipError_t launch_scan_sq8sym(const ScanSymArgs &a, int B, hipStream_t st) {
+    if (a.n_items == 0) return hipSuccess;
+    uint64_t grid = a.n_items;
+    if (a.queue) {
+        int dev = 0, cus = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        grid = std::min<uint64_t>((uint64_t)cus * 8, a.n_items);
+    }
+    switch (B) {
+    case 256: hipLaunchKernelGGL(scan_sym_kernel<256>, dim3((uint32_t)grid), dim3(64), 0, st, a); break;
+    case 512: hipLaunchKernelGGL(scan_sym_kernel<512>, dim3((uint32_t)grid), dim3(64), 0, st, a); break;
+    case 1024: hipLaunchKernelGGL(scan_sym_kernel<1024>, dim3((uint32_t)grid), dim3(64), 0, st, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace fbf
