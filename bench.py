@@ -100,11 +100,15 @@ def main() -> int:
     stats = ds.last_stats()
     n_scan, ms_scan = ds.profile_collect("scan")
     n_merge, ms_merge = ds.profile_collect("merge")
+    n_rr, ms_rr = ds.profile_collect("rerank")
+    n_merge += n_rr
+    ms_merge += ms_rr
     ds.profile_collect("all")
     ms_step = 1000.0 * elapsed / a.steps
     qps = nq * a.steps / elapsed
     log(f"{a.steps} steps in {elapsed:.3f}s: {qps:.0f} QPS; scan {ms_scan / max(1, a.steps):.1f} ms/step "
-        f"({n_scan} launches), merge {ms_merge / max(1, a.steps):.1f} ms/step")
+        f"({n_scan} launches), merge+rerank {ms_merge / max(1, a.steps):.1f} ms/step, "
+        f"{stats['work_items']} work items, {stats['pairs_computed'] / max(1, stats['pairs']):.3f}x pairs computed")
 
     ids = out[0].cpu().numpy().view(np.uint32)
     dists = out[1].cpu().numpy()
@@ -128,6 +132,8 @@ def main() -> int:
 
     # Roofline of the dominant kernel (the scan), from its measured launches.
     pairs = stats["pairs"]
+    # Launches of one step can differ in size (one per sorted layout for the
+    # VALU scans); the achieved rate is total algorithmic work / total time.
     avg_launch_ms = ms_scan / max(1, n_scan)
     launches_per_step = max(1, n_scan // a.steps)
     pairs_per_launch = pairs / launches_per_step

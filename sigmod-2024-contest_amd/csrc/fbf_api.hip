@@ -81,8 +81,21 @@ struct HostPinned {
 };
 
 struct LayoutDev {
-    uint32_t *ids = nullptr;   // [n_pad] position -> original id (pad: kPadId)
-    float *soa = nullptr;      // FP32 scan copy
+    uint32_t *ids = nullptr;      // [n_pad] position -> original id (pad: kPadId)
+    float *soa = nullptr;         // FP32 scan copy (exact mode)
+    void *bf16 = nullptr;         // BF16 MFMA row fragments
+    void *fp8 = nullptr;          // FP8 e4m3 MFMA row fragments
+    uint8_t *fp8_scale = nullptr; // [n_pad] E8M0 row scales
+    float *xnorm = nullptr;       // [n_pad] |x|^2 per position (FP8)
+    void *sq8 = nullptr;          // uint8 codes in MFMA order
+    float *sq8_norm = nullptr;    // [n_pad] |xhat|^2 per position
+    int8_t *sym = nullptr;        // [n_pad][112] symmetric int8 codes
+    void free_all() {
+        for (void *p : {(void *)ids, (void *)soa, bf16, fp8, (void *)fp8_scale, (void *)xnorm, sq8,
+                        (void *)sq8_norm, (void *)sym})
+            if (p) (void)hipFree(p);
+        *this = LayoutDev();
+    }
 };
 
 struct ProfEvent {
@@ -114,6 +127,9 @@ struct fbf_dataset {
     // device data
     float *base_rows = nullptr;  // [n][d] original order
     LayoutDev lay[2];
+    float *sq8_mi = nullptr, *sq8_dif = nullptr, *sq8_w = nullptr;  // [d] each
+    float alpha = 0.f;           // symmetric int8 quantiser scale
+    std::vector<float> h_mi, h_dif;
     Scratch scr;
     std::mutex mu;
     // measurement
@@ -219,15 +235,77 @@ static fbf_status dataset_build(fbf_dataset *ds, const float *labels, const floa
         p.resize(n_pad, kPadId);
         HIP_TRY(hipMalloc(&ds->lay[L].ids, n_pad * sizeof(uint32_t)));
         HIP_TRY(hipMemcpy(ds->lay[L].ids, p.data(), n_pad * sizeof(uint32_t), hipMemcpyHostToDevice));
-        if (ds->repr & FBF_REPR_F32) {
-            HIP_TRY(hipMalloc(&ds->lay[L].soa, n_pad * kDim * sizeof(float)));
-            ProfEvent ev;
-            record_begin(ds, 3, st, ev);
-            HIP_TRY(launch_build_soa(ds->base_rows, ds->lay[L].ids, n, ds->lay[L].soa, st));
-            record_end(ds, st, ev);
+    }
+    ProfEvent ev;
+    record_begin(ds, 3, st, ev);
+    const uint32_t repr = ds->repr;
+    const uint64_t nb32 = n_pad / 32;  // 32-row MFMA blocks covering the padded layout
+    DevBuf norms, tmp_codes, tmp_xhat, absbits;
+    if (repr & (FBF_REPR_BF16 | FBF_REPR_FP8)) {
+        HIP_TRY(norms.reserve(n * sizeof(float)));
+        HIP_TRY(launch_row_norms(ds->base_rows, n, norms.as<float>(), st));
+    }
+    if (repr & FBF_REPR_SQ8) {
+        HIP_TRY(hipMalloc(&ds->sq8_mi, kDim * sizeof(float)));
+        HIP_TRY(hipMalloc(&ds->sq8_dif, kDim * sizeof(float)));
+        HIP_TRY(hipMalloc(&ds->sq8_w, kDim * sizeof(float)));
+        HIP_TRY(tmp_codes.reserve(n * kDim));
+        HIP_TRY(tmp_xhat.reserve(n * sizeof(float)));
+        HIP_TRY(launch_sq8_train_encode(ds->base_rows, n, ds->sq8_mi, ds->sq8_dif, tmp_codes.as<uint8_t>(),
+                                        tmp_xhat.as<float>(), st));
+        ds->h_mi.resize(kDim);
+        ds->h_dif.resize(kDim);
+        HIP_TRY(hipMemcpyAsync(ds->h_mi.data(), ds->sq8_mi, kDim * 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(ds->h_dif.data(), ds->sq8_dif, kDim * 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        std::vector<float> w(kDim);
+        for (int k = 0; k < kDim; ++k) w[k] = ds->h_dif[k] / 255.0f;
+        HIP_TRY(hipMemcpy(ds->sq8_w, w.data(), kDim * 4, hipMemcpyHostToDevice));
+    }
+    if (repr & FBF_REPR_SQ8SYM) {
+        HIP_TRY(absbits.reserve(sizeof(uint32_t)));
+        HIP_TRY(hipMemsetAsync(absbits.p, 0, sizeof(uint32_t), st));
+        HIP_TRY(launch_absmax(ds->base_rows, n * kDim, absbits.as<uint32_t>(), st));
+        uint32_t bits = 0;
+        HIP_TRY(hipMemcpyAsync(&bits, absbits.p, 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        std::memcpy(&ds->alpha, &bits, 4);
+        if (!(ds->alpha > 0.f)) return fail(FBF_ERR_INVALID_ARG, "all-zero base: symmetric int8 scale undefined");
+    }
+    for (int L = 0; L < 2; ++L) {
+        LayoutDev &D = ds->lay[L];
+        if (repr & FBF_REPR_F32) {
+            HIP_TRY(hipMalloc(&D.soa, n_pad * kDim * sizeof(float)));
+            HIP_TRY(launch_build_soa(ds->base_rows, D.ids, n, D.soa, st));
+        }
+        if (repr & FBF_REPR_BF16) {
+            HIP_TRY(hipMalloc(&D.bf16, nb32 * 7 * 64 * 16));
+            HIP_TRY(launch_build_mfma(0, ds->base_rows, D.ids, norms.as<float>(), nullptr, n, D.bf16, nullptr, st));
+        }
+        if (repr & FBF_REPR_FP8) {
+            HIP_TRY(hipMalloc(&D.fp8, nb32 * 2 * 64 * 32));
+            HIP_TRY(hipMalloc(&D.fp8_scale, n_pad));
+            HIP_TRY(hipMalloc(&D.xnorm, n_pad * sizeof(float)));
+            HIP_TRY(launch_build_mfma(1, ds->base_rows, D.ids, nullptr, nullptr, n, D.fp8, D.fp8_scale, st));
+            HIP_TRY(launch_gather_f32(norms.as<float>(), D.ids, n, n_pad, D.xnorm, st));
+        }
+        if (repr & FBF_REPR_SQ8) {
+            HIP_TRY(hipMalloc(&D.sq8, nb32 * 7 * 64 * 8));
+            HIP_TRY(hipMalloc(&D.sq8_norm, n_pad * sizeof(float)));
+            HIP_TRY(launch_build_mfma(2, nullptr, D.ids, nullptr, tmp_codes.as<uint8_t>(), n, D.sq8, nullptr, st));
+            HIP_TRY(launch_gather_f32(tmp_xhat.as<float>(), D.ids, n, n_pad, D.sq8_norm, st));
+        }
+        if (repr & FBF_REPR_SQ8SYM) {
+            HIP_TRY(hipMalloc(&D.sym, n_pad * 112));
+            HIP_TRY(launch_sym_encode(ds->base_rows, D.ids, n, n_pad, ds->alpha, D.sym, st));
         }
     }
+    record_end(ds, st, ev);
     HIP_TRY(hipStreamSynchronize(st));
+    norms.release();
+    tmp_codes.release();
+    tmp_xhat.release();
+    absbits.release();
     return FBF_OK;
 }
 
@@ -280,11 +358,10 @@ extern "C" fbf_status fbf_dataset_create_device(const float *vecs_dev, const flo
 extern "C" fbf_status fbf_dataset_destroy(fbf_dataset *ds) {
     if (!ds) return FBF_OK;
     hipDeviceSynchronize();
-    if (ds->base_rows) hipFree(ds->base_rows);
-    for (auto &L : ds->lay) {
-        if (L.ids) hipFree(L.ids);
-        if (L.soa) hipFree(L.soa);
-    }
+    if (ds->base_rows) (void)hipFree(ds->base_rows);
+    for (auto &L : ds->lay) L.free_all();
+    for (float *p : {ds->sq8_mi, ds->sq8_dif, ds->sq8_w})
+        if (p) (void)hipFree(p);
     ds->scr.release();
     for (auto &ev : ds->events) {
         hipEventDestroy(ev.a);
@@ -458,9 +535,22 @@ static MergeArgs merge_args(const Scratch &S, int B, uint64_t nq, uint32_t k, ui
 }
 
 // ---------------------------------------------------------------- search
-static uint32_t default_chunk(const fbf_dataset *ds) {
-    (void)ds;
-    return 256 * 1024;
+// Work items in descending cost order (largest first), stable within a layout
+// run: big items start early and the small ones fill the tail.
+static void order_items(Plan &P, bool by_layout) {
+    std::stable_sort(P.items.begin(), P.items.end(), [&](const WorkItem &x, const WorkItem &y) {
+        if (by_layout && x.layout != y.layout) return x.layout < y.layout;
+        const uint64_t cx = (uint64_t)(x.row_end - x.row_begin) * x.q_count;
+        const uint64_t cy = (uint64_t)(y.row_end - y.row_begin) * y.q_count;
+        return cx > cy;
+    });
+}
+
+static fbf_status mode_check(const fbf_dataset *ds, uint32_t mode) {
+    static const uint32_t need[5] = {FBF_REPR_F32, FBF_REPR_SQ8, FBF_REPR_BF16, FBF_REPR_FP8, FBF_REPR_SQ8SYM};
+    if (mode > FBF_MODE_SQ8SYM) return fail(FBF_ERR_INVALID_ARG, "unknown mode");
+    if (!(ds->repr & need[mode])) return fail(FBF_ERR_NO_REPR, "dataset was created without this mode's representation");
+    return FBF_OK;
 }
 
 extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, const float *metas,
@@ -472,55 +562,113 @@ extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, cons
     if (nq == 0) return FBF_OK;
     fbf_search_opts o{};
     if (opts) o = *opts;
-    if (o.mode != FBF_MODE_EXACT_F32) return fail(FBF_ERR_UNSUPPORTED, "mode not available yet");
-    if (!(ds->repr & FBF_REPR_F32)) return fail(FBF_ERR_NO_REPR, "dataset has no FP32 scan copy");
+    fbf_status s = mode_check(ds, o.mode);
+    if (s != FBF_OK) return s;
+    const bool exact = o.mode == FBF_MODE_EXACT_F32;
+    const bool mfma = o.mode == FBF_MODE_BF16 || o.mode == FBF_MODE_FP8 || o.mode == FBF_MODE_SQ8;
+    // Kept depth per list: k for the exact scan, k' (over-fetch) otherwise.
+    uint32_t K = k;
+    if (!exact) {
+        K = o.kprime ? o.kprime : 4 * k;
+        if (K < k) return fail(FBF_ERR_INVALID_ARG, "kprime must be >= k");
+        if (K > 896) return fail(FBF_ERR_INVALID_ARG, "kprime must be <= 896");
+    }
     const uint32_t chunk = o.chunk_rows ? (o.chunk_rows + kRowTile - 1) / kRowTile * kRowTile
-                                        : default_chunk(ds);
-    const uint32_t gmax = o.group_queries ? std::min<uint32_t>(o.group_queries, kMaxGroup) : kMaxGroup;
+                                        : (mfma ? 1024u * 1024u : 256u * 1024u);
+    const uint32_t gcap = mfma ? 64u : (uint32_t)kMaxGroup;
+    const uint32_t gmax = o.group_queries ? std::min<uint32_t>(o.group_queries, gcap) : gcap;
     hipStream_t st = (hipStream_t)stream;
 
     std::lock_guard<std::mutex> lock(ds->mu);
     HIP_TRY(hipSetDevice(ds->device));
     std::vector<QRange> qr;
-    fbf_status s = compute_ranges(ds, metas, nq, qr);
+    s = compute_ranges(ds, metas, nq, qr);
     if (s != FBF_OK) return s;
     Plan P;
     make_plan(qr, chunk, gmax, P);
+    order_items(P, !mfma);
     ds->last_pairs = P.pairs;
     ds->last_pairs_computed = P.pairs_computed;
     ds->last_items = P.items.size();
 
-    const uint32_t K = k;
     const int B = list_capacity((int)K);
     Scratch &S = ds->scr;
     s = upload_plan(S, P, nq, B, st);
     if (s != FBF_OK) return s;
+    uint32_t *queue = nullptr;
+    if (o.persistent) queue = S.d_queue.as<uint32_t>();
 
     // Scan.  Each work item writes every list slot it owns (count and keys),
     // so no clearing is needed.
-    for (uint32_t L = 0; L < 2; ++L) {
-        uint64_t i0, i1;
-        layout_run(P, L, i0, i1);
-        if (i1 == i0) continue;
-        ScanArgs a{};
-        a.items = S.d_items.as<WorkItem>() + i0;
-        a.n_items = i1 - i0;
+    if (mfma) {
+        const int variant = o.mode == FBF_MODE_BF16 ? 0 : o.mode == FBF_MODE_FP8 ? 1 : 2;
+        ScanMfmaArgs a{};
+        a.items = S.d_items.as<WorkItem>();
+        a.n_items = P.items.size();
         a.iq = S.d_iq.as<ItemQuery>();
         a.queries = queries_dev;
-        a.soa = ds->lay[L].soa;
-        a.ids = ds->lay[L].ids;
+        a.sq8_w = ds->sq8_w;
+        for (int L = 0; L < 2; ++L) {
+            const LayoutDev &D = ds->lay[L];
+            a.lay[L].ids = D.ids;
+            a.lay[L].tiles = variant == 0 ? D.bf16 : variant == 1 ? D.fp8 : D.sq8;
+            a.lay[L].xnorm = variant == 1 ? D.xnorm : D.sq8_norm;
+            a.lay[L].scale = D.fp8_scale;
+        }
         a.lists = S.d_lists.as<uint64_t>();
         a.counts = S.d_counts.as<uint32_t>();
         a.K = K;
-        a.queue = nullptr;
-        if (o.persistent) {
-            HIP_TRY(hipMemsetAsync(S.d_queue.p, 0, sizeof(uint32_t), st));
-            a.queue = S.d_queue.as<uint32_t>();
-        }
+        a.queue = queue;
+        if (queue) HIP_TRY(launch_fill_u32(queue, 0u, 1, st));
+        const int qb = gmax > 32 ? 2 : 1;
         ProfEvent ev;
         record_begin(ds, 0, st, ev);
-        HIP_TRY(launch_scan_f32(a, B, st));
+        HIP_TRY(launch_scan_mfma(variant, a, B, qb, st));
         record_end(ds, st, ev);
+    } else {
+        int8_t *qcodes = nullptr;
+        if (o.mode == FBF_MODE_SQ8SYM) {
+            HIP_TRY(S.d_q.reserve(nq * 112));
+            qcodes = S.d_q.as<int8_t>();
+            HIP_TRY(launch_sym_encode(queries_dev, nullptr, nq, nq, ds->alpha, qcodes, st));
+        }
+        for (uint32_t L = 0; L < 2; ++L) {
+            uint64_t i0, i1;
+            layout_run(P, L, i0, i1);
+            if (i1 == i0) continue;
+            if (queue) HIP_TRY(launch_fill_u32(queue, 0u, 1, st));
+            ProfEvent ev;
+            record_begin(ds, 0, st, ev);
+            if (exact) {
+                ScanArgs a{};
+                a.items = S.d_items.as<WorkItem>() + i0;
+                a.n_items = i1 - i0;
+                a.iq = S.d_iq.as<ItemQuery>();
+                a.queries = queries_dev;
+                a.soa = ds->lay[L].soa;
+                a.ids = ds->lay[L].ids;
+                a.lists = S.d_lists.as<uint64_t>();
+                a.counts = S.d_counts.as<uint32_t>();
+                a.K = K;
+                a.queue = queue;
+                HIP_TRY(launch_scan_f32(a, B, st));
+            } else {
+                ScanSymArgs a{};
+                a.items = S.d_items.as<WorkItem>() + i0;
+                a.n_items = i1 - i0;
+                a.iq = S.d_iq.as<ItemQuery>();
+                a.qcodes = qcodes;
+                a.codes = ds->lay[L].sym;
+                a.ids = ds->lay[L].ids;
+                a.n_rows = ds->n;
+                a.lists = S.d_lists.as<uint64_t>();
+                a.counts = S.d_counts.as<uint32_t>();
+                a.K = K;
+                a.queue = queue;
+                HIP_TRY(launch_scan_sq8sym(a, B, st));
+            }
+            record_end(ds, st, ev);
+        }
     }
 
     MergeArgs m = merge_args(S, B, nq, k, K);
@@ -528,10 +676,10 @@ extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, cons
     m.base_rows = ds->base_rows;
     m.out_ids = out_ids_dev;
     m.out_dists = out_dists_dev;
-    m.rerank = 0;
+    m.rerank = exact ? 0 : 1;
     m.key_int = 0;
     ProfEvent ev;
-    record_begin(ds, 1, st, ev);
+    record_begin(ds, exact ? 1 : 2, st, ev);
     HIP_TRY(launch_merge(m, st));
     record_end(ds, st, ev);
     return FBF_OK;
@@ -567,8 +715,17 @@ extern "C" fbf_status fbf_search_host(fbf_dataset *ds, const float *queries, con
 }
 
 extern "C" fbf_status fbf_dataset_quant_params(const fbf_dataset *ds, float *alpha, float *mi, float *dif) {
-    (void)ds; (void)alpha; (void)mi; (void)dif;
-    return fail(FBF_ERR_NO_REPR, "quantisers not built");
+    if (!ds) return fail(FBF_ERR_INVALID_ARG, "null dataset");
+    if (alpha) {
+        if (!(ds->repr & FBF_REPR_SQ8SYM)) return fail(FBF_ERR_NO_REPR, "dataset has no symmetric int8 codes");
+        *alpha = ds->alpha;
+    }
+    if (mi || dif) {
+        if (!(ds->repr & FBF_REPR_SQ8)) return fail(FBF_ERR_NO_REPR, "dataset has no uint8 codes");
+        if (mi) std::memcpy(mi, ds->h_mi.data(), kDim * sizeof(float));
+        if (dif) std::memcpy(dif, ds->h_dif.data(), kDim * sizeof(float));
+    }
+    return FBF_OK;
 }
 
 // ---------------------------------------------------------------- measurement
@@ -620,7 +777,7 @@ extern "C" fbf_status fbf_last_search_stats(const fbf_dataset *ds, uint64_t *pai
 
 extern "C" const char *fbf_last_error(void) { return g_err.c_str(); }
 
-extern "C" const char *fbf_version(void) { return "fbf 0.1 gfx950: exact-f32 sq8sym"; }
+extern "C" const char *fbf_version(void) { return "fbf 0.2 gfx950: exact-f32 bf16-mfma fp8-mfma sq8-mfma sq8sym"; }
 
 // ---------------------------------------------------------------- reference-shaped entry points
 //

@@ -100,6 +100,27 @@ struct ScanSymArgs {
     uint32_t *queue;
 };
 
+// One sorted layout as seen by the MFMA scans.
+struct LayoutMfma {
+    const void *tiles;       // row fragments in MFMA order (variant encoding)
+    const float *xnorm;      // [n_pad32] |x|^2 (FP8) or |xhat|^2 (SQ8) per position
+    const uint8_t *scale;    // [n_pad32] E8M0 row scale (FP8)
+    const uint32_t *ids;     // position -> original id
+};
+
+struct ScanMfmaArgs {
+    const WorkItem *items;
+    uint64_t n_items;
+    const ItemQuery *iq;
+    const float *queries;    // [nq][kDim] FP32
+    const float *sq8_w;      // [kDim] dif_k / 255 (SQ8)
+    LayoutMfma lay[2];
+    uint64_t *lists;
+    uint32_t *counts;
+    uint32_t K;
+    uint32_t *queue;
+};
+
 struct MergeArgs {
     const uint64_t *slot_off;  // [nq + 1] CSR over q_slots
     const uint64_t *q_slots;   // slot indices
@@ -118,8 +139,21 @@ struct MergeArgs {
 };
 
 hipError_t launch_scan_f32(const ScanArgs &a, int B, hipStream_t st);
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st);
 hipError_t launch_merge(const MergeArgs &a, hipStream_t st);
 hipError_t launch_scan_sq8sym(const ScanSymArgs &a, int B, hipStream_t st);
+// variant: 0 BF16, 1 FP8, 2 SQ8; qb: query blocks of 32 per work item (1 or 2)
+hipError_t launch_scan_mfma(int variant, const ScanMfmaArgs &a, int B, int qb, hipStream_t st);
+hipError_t launch_build_mfma(int variant, const float *rows, const uint32_t *perm, const float *norms,
+                             const uint8_t *sq8_codes, uint64_t n, void *out, uint8_t *scale, hipStream_t st);
+hipError_t launch_row_norms(const float *rows, uint64_t n, float *out, hipStream_t st);
+hipError_t launch_gather_f32(const float *src, const uint32_t *perm, uint64_t n, uint64_t n_pad, float *dst,
+                             hipStream_t st);
+hipError_t launch_absmax(const float *x, uint64_t count, uint32_t *out_bits, hipStream_t st);
+hipError_t launch_sym_encode(const float *rows, const uint32_t *perm, uint64_t n, uint64_t n_out, float alpha,
+                             int8_t *out, hipStream_t st);
+hipError_t launch_sq8_train_encode(const float *rows, uint64_t n, float *mi, float *dif, uint8_t *codes,
+                                   float *xhat_norm, hipStream_t st);
 hipError_t launch_build_soa(const float *rows, const uint32_t *perm, uint64_t n,
                             float *soa, hipStream_t st);
 

@@ -45,6 +45,19 @@ hipError_t launch_build_soa(const float *rows, const uint32_t *perm, uint64_t n,
     return hipGetLastError();
 }
 
+// Fill kernel for counters (used instead of a memset so the search can be
+// captured in a graph).
+__global__ void fill_u32_kernel(uint32_t *p, uint32_t v, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, p, v, n);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------- exact scan
 
 

@@ -1,0 +1,568 @@
+// fbf_scan_mfma.hip -- approximate distance scans on the matrix cores.
+//
+// The q.x term of |x|^2 - 2 q.x (the GEMM form of the reference's commented-out
+// exact_knn, bruteforce.h:129-198) runs as a 32x32 MFMA tile: 32 candidate rows
+// (A operand) against 32 queries of the work item (B operand), FP32
+// accumulation, with the row norm in FP32.  Each lane of the output tile holds
+// one query (column l % 32) and 16 rows, so the threshold test of the top-k'
+// selection is one compare per score against that lane's own threshold.
+// Candidates that pass go to the query's list; the merge kernel then reranks
+// the k' best per query with the exact FP32 distance.
+//
+// Variants (one template, three row encodings):
+//   BF16: rows as bf16, |x|^2 folded into two spare K slots (hi + lo bf16)
+//         with 1.0 in the matching query slots: score = acc.
+//   FP8:  rows e4m3 with a power-of-two scale per row, queries e4m3 with a
+//         power-of-two scale per query; both scales ride in the MFMA's E8M0
+//         scale operands; score = acc + |x|^2 (FP32, epilogue).
+//   SQ8:  per-dimension uint8 codes (SQ8Quantizer, sq8_quant.hpp:30-63)
+//         dequantised in registers to bf16 (codes 0..255 are exact in bf16);
+//         the query side carries dif_i / 255; score = acc + |xhat|^2.
+//
+// MFMA lane layouts (measured on gfx950):
+//   v_mfma_f32_32x32x16_bf16: A lane l holds row l%32, k = 8*(l/32) + e;
+//     B lane l holds column l%32, same k; D element j of lane l is
+//     row 8*(j/4) + 4*(l/32) + j%4, column l%32.
+//   v_mfma_scale_f32_32x32x64_f8f6f4 (e4m3): lane l holds row / column l%32,
+//     k = 32*(l/32) + e, e = 0..31; same D layout.
+#include "fbf_select.hpp"
+
+namespace fbf {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef int i32x8 __attribute__((ext_vector_type(8)));
+
+constexpr int kMTile = 32;     // rows per MFMA tile
+constexpr int kBfSteps = 7;    // K = 112 = 7 x 16 (100 dims + 2 norm slots + pad)
+constexpr int kF8Steps = 2;    // K = 128 = 2 x 64
+
+__device__ inline uint16_t bf16_bits(float f) {  // round to nearest even
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ inline float bf16_to_f(uint16_t h) { return __builtin_bit_cast(float, (uint32_t)h << 16); }
+
+// ---------------------------------------------------------------- row encodings (dataset build)
+
+// BF16 tiles: block rb of 32 rows, step s, lane l: 8 bf16 = row rb*32 + l%32,
+// k = 16 s + 8 (l/32) + e.  k < 100: x_k; k = 100 / 101: hi / lo of |x|^2.
+__global__ void build_bf16_kernel(const float *__restrict__ rows, const uint32_t *__restrict__ perm,
+                                  const float *__restrict__ norms, uint64_t n, uint64_t n_blocks,
+                                  uint4 *__restrict__ out) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_blocks * kBfSteps * 64) return;
+    const uint32_t l = t % 64, s = (t / 64) % kBfSteps;
+    const uint64_t rb = t / (64 * kBfSteps);
+    const uint64_t p = rb * kMTile + l % 32;
+    uint16_t h[8];
+    for (int e = 0; e < 8; ++e) {
+        const int k = 16 * s + 8 * (l / 32) + e;
+        float v = 0.f;
+        if (p < n) {
+            const uint32_t id = perm[p];
+            if (k < kDim) v = rows[(uint64_t)id * kDim + k];
+            else if (k == kDim) v = norms[id];
+        }
+        uint16_t b = bf16_bits(v);
+        if (p < n && k == kDim + 1) {
+            const float nn = norms[perm[p]];
+            b = bf16_bits(nn - bf16_to_f(bf16_bits(nn)));
+        }
+        h[e] = b;
+    }
+    out[t] = make_uint4(h[0] | (uint32_t)h[1] << 16, h[2] | (uint32_t)h[3] << 16,
+                        h[4] | (uint32_t)h[5] << 16, h[6] | (uint32_t)h[7] << 16);
+}
+
+// Power-of-two scale for e4m3: |x| / 2^e lands in (128, 256] for the largest
+// element.  Returns the exponent e (0 for an all-zero vector).
+__device__ inline int fp8_exponent(float absmax) {
+    if (!(absmax > 0.f)) return 0;
+    int ex;
+    frexpf(absmax, &ex);  // absmax = m * 2^ex, m in [0.5, 1)
+    return ex - 8;
+}
+
+__device__ inline uint32_t pack_fp8x4(float a, float b, float c, float d) {
+    int w = 0;
+    w = __builtin_amdgcn_cvt_pk_fp8_f32(a, b, w, false);
+    w = __builtin_amdgcn_cvt_pk_fp8_f32(c, d, w, true);
+    return (uint32_t)w;
+}
+
+// FP8 tiles: block rb, step s (K 64), lane l: 32 bytes = row rb*32 + l%32,
+// k = 64 s + 32 (l/32) + e; values x_k / 2^ex.  scale[p] = 127 + ex (E8M0).
+__global__ void build_fp8_kernel(const float *__restrict__ rows, const uint32_t *__restrict__ perm,
+                                 uint64_t n, uint64_t n_blocks, uint4 *__restrict__ out,
+                                 uint8_t *__restrict__ scale) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_blocks * kF8Steps * 64) return;
+    const uint32_t l = t % 64, s = (t / 64) % kF8Steps;
+    const uint64_t rb = t / (64 * kF8Steps);
+    const uint64_t p = rb * kMTile + l % 32;
+    uint32_t w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int ex = 0;
+    if (p < n) {
+        const float *x = rows + (uint64_t)perm[p] * kDim;
+        float am = 0.f;
+        for (int k = 0; k < kDim; ++k) am = fmaxf(am, fabsf(x[k]));
+        ex = fp8_exponent(am);
+        float v[32];
+        for (int e = 0; e < 32; ++e) {
+            const int k = 64 * s + 32 * (l / 32) + e;
+            v[e] = k < kDim ? ldexpf(x[k], -ex) : 0.f;
+        }
+        for (int q = 0; q < 8; ++q) w[q] = pack_fp8x4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+    }
+    out[2 * t] = make_uint4(w[0], w[1], w[2], w[3]);
+    out[2 * t + 1] = make_uint4(w[4], w[5], w[6], w[7]);
+    if (s == 0 && l < 32) scale[p] = (uint8_t)(127 + ex);
+}
+
+// SQ8 codes in MFMA order: block rb, step s, lane l: 8 bytes = codes of row
+// rb*32 + l%32 at k = 16 s + 8 (l/32) + e (zero beyond d).
+__global__ void build_sq8_kernel(const uint8_t *__restrict__ codes_orig, const uint32_t *__restrict__ perm,
+                                 uint64_t n, uint64_t n_blocks, uint2 *__restrict__ out) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_blocks * kBfSteps * 64) return;
+    const uint32_t l = t % 64, s = (t / 64) % kBfSteps;
+    const uint64_t rb = t / (64 * kBfSteps);
+    const uint64_t p = rb * kMTile + l % 32;
+    uint32_t lo = 0, hi = 0;
+    if (p < n) {
+        const uint8_t *c = codes_orig + (uint64_t)perm[p] * kDim;
+        for (int e = 0; e < 8; ++e) {
+            const int k = 16 * s + 8 * (l / 32) + e;
+            const uint32_t v = k < kDim ? c[k] : 0u;
+            if (e < 4) lo |= v << (8 * e); else hi |= v << (8 * (e - 4));
+        }
+    }
+    out[t] = make_uint2(lo, hi);
+}
+
+// Per-position FP32 values gathered into a layout (row norms).
+__global__ void gather_f32_kernel(const float *__restrict__ src, const uint32_t *__restrict__ perm,
+                                  uint64_t n, uint64_t n_pad, float *__restrict__ dst) {
+    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_pad) return;
+    dst[p] = p < n ? src[perm[p]] : 0.f;
+}
+
+// |x|^2 of each original row (FP32, any order: these feed approximate scores).
+__global__ void row_norms_kernel(const float *__restrict__ rows, uint64_t n, float *__restrict__ out) {
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const float4 *x = reinterpret_cast<const float4 *>(rows + r * kDim);
+    float s = 0.f;
+    for (int g = 0; g < kDim / 4; ++g) {
+        const float4 v = x[g];
+        s = __builtin_fmaf(v.x, v.x, s);
+        s = __builtin_fmaf(v.y, v.y, s);
+        s = __builtin_fmaf(v.z, v.z, s);
+        s = __builtin_fmaf(v.w, v.w, s);
+    }
+    out[r] = s;
+}
+
+// Per-dimension min / max for SQ8 (SQ8Quantizer::train, sq8_quant.hpp:30-47):
+// one thread per dimension, strided over rows (a build step, not the hot path).
+__global__ void sq8_minmax_kernel(const float *__restrict__ rows, uint64_t n, float *__restrict__ mi,
+                                  float *__restrict__ mx) {
+    __shared__ float smi[256], smx[256];
+    const int dim = blockIdx.x;
+    float a = __builtin_inff(), b = -__builtin_inff();
+    for (uint64_t r = threadIdx.x; r < n; r += blockDim.x) {
+        const float v = rows[r * kDim + dim];
+        a = fminf(a, v);
+        b = fmaxf(b, v);
+    }
+    smi[threadIdx.x] = a;
+    smx[threadIdx.x] = b;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            smi[threadIdx.x] = fminf(smi[threadIdx.x], smi[threadIdx.x + s]);
+            smx[threadIdx.x] = fmaxf(smx[threadIdx.x], smx[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        mi[dim] = smi[0];
+        mx[dim] = smx[0];
+    }
+}
+
+// SQ8Quantizer::encode (sq8_quant.hpp:51-63): trunc(clamp((x-mi)/dif, 0, 1)*255);
+// a constant dimension (dif = 0) encodes to 0.  Also |xhat|^2 of the
+// dequantised row xhat_i = mi_i + (c_i + 0.5) dif_i / 255 (L2SqrSQ8_ext's
+// reconstruction, distance.hpp:284-288).
+__global__ void sq8_encode_kernel(const float *__restrict__ rows, uint64_t n, const float *__restrict__ mi,
+                                  const float *__restrict__ dif, uint8_t *__restrict__ codes,
+                                  float *__restrict__ xhat_norm) {
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    float s = 0.f;
+    for (int k = 0; k < kDim; ++k) {
+        uint8_t c = 0;
+        if (dif[k] > 0.f) {
+            float v = (rows[r * kDim + k] - mi[k]) / dif[k];
+            v = fminf(fmaxf(v, 0.f), 1.f);
+            c = (uint8_t)(v * 255.0f);
+        }
+        codes[r * kDim + k] = c;
+        const float xh = mi[k] + ((float)c + 0.5f) * dif[k] * (1.0f / 255.0f);
+        s = __builtin_fmaf(xh, xh, s);
+    }
+    xhat_norm[r] = s;
+}
+
+hipError_t launch_build_mfma(int variant, const float *rows, const uint32_t *perm, const float *norms,
+                             const uint8_t *sq8_codes, uint64_t n, void *out, uint8_t *scale,
+                             hipStream_t st) {
+    const uint64_t n_blocks = (n + kMTile - 1) / kMTile;
+    if (variant == 0) {
+        const uint64_t total = n_blocks * kBfSteps * 64;
+        hipLaunchKernelGGL(build_bf16_kernel, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, st, rows,
+                           perm, norms, n, n_blocks, (uint4 *)out);
+    } else if (variant == 1) {
+        const uint64_t total = n_blocks * kF8Steps * 64;
+        hipLaunchKernelGGL(build_fp8_kernel, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, st, rows,
+                           perm, n, n_blocks, (uint4 *)out, scale);
+    } else {
+        const uint64_t total = n_blocks * kBfSteps * 64;
+        hipLaunchKernelGGL(build_sq8_kernel, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, st,
+                           sq8_codes, perm, n, n_blocks, (uint2 *)out);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_row_norms(const float *rows, uint64_t n, float *out, hipStream_t st) {
+    hipLaunchKernelGGL(row_norms_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, rows, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_f32(const float *src, const uint32_t *perm, uint64_t n, uint64_t n_pad, float *dst,
+                             hipStream_t st) {
+    hipLaunchKernelGGL(gather_f32_kernel, dim3((uint32_t)((n_pad + 255) / 256)), dim3(256), 0, st, src, perm, n,
+                       n_pad, dst);
+    return hipGetLastError();
+}
+
+__global__ void sq8_dif_kernel(const float *__restrict__ mi, float *__restrict__ dif) {
+    const int k = threadIdx.x;
+    if (k < kDim) dif[k] = dif[k] - mi[k];  // dif held the per-dimension max
+}
+
+// SQ8 training and encoding on the device: mi, dif (d floats each), codes
+// [n][d] in original row order and |xhat|^2 per original row.
+hipError_t launch_sq8_train_encode(const float *rows, uint64_t n, float *mi, float *dif, uint8_t *codes,
+                                   float *xhat_norm, hipStream_t st) {
+    hipLaunchKernelGGL(sq8_minmax_kernel, dim3(kDim), dim3(256), 0, st, rows, n, mi, dif);
+    hipLaunchKernelGGL(sq8_dif_kernel, dim3(1), dim3(128), 0, st, (const float *)mi, dif);
+    hipLaunchKernelGGL(sq8_encode_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, rows, n,
+                       (const float *)mi, (const float *)dif, codes, xhat_norm);
+    return hipGetLastError();
+}
+
+
+// ---------------------------------------------------------------- the scan
+
+template <int VAR> struct RowFrag;
+template <> struct RowFrag<0> { bf16x8 v[kBfSteps]; };                 // BF16
+template <> struct RowFrag<1> { i32x8 v[kF8Steps]; int scale; };      // FP8
+template <> struct RowFrag<2> { bf16x8 v[kBfSteps]; };                 // SQ8 (dequantised)
+template <int VAR> struct QFrag;
+template <> struct QFrag<0> { bf16x8 v[kBfSteps]; };
+template <> struct QFrag<1> { i32x8 v[kF8Steps]; int scale; };
+template <> struct QFrag<2> { bf16x8 v[kBfSteps]; };
+
+__device__ inline bf16x8 bf16x8_from_bits(uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    u32x4 u = {a, b, c, d};
+    return __builtin_bit_cast(bf16x8, u);
+}
+
+// bf16 bits of an exact small integer given as float: the top half.
+__device__ inline uint32_t pack_bf16_exact(float lo, float hi) {
+    return (__builtin_bit_cast(uint32_t, lo) >> 16) | (__builtin_bit_cast(uint32_t, hi) & 0xFFFF0000u);
+}
+
+template <int VAR>
+__device__ inline void load_rows(const ScanMfmaArgs &a, const LayoutMfma &L, uint32_t t, uint32_t lane,
+                                 RowFrag<VAR> &r) {
+    const uint64_t rb = t / kMTile;
+    if constexpr (VAR == 0) {
+        const uint4 *src = reinterpret_cast<const uint4 *>(L.tiles) + rb * kBfSteps * 64 + lane;
+#pragma unroll
+        for (int s = 0; s < kBfSteps; ++s) {
+            const uint4 u = src[s * 64];
+            r.v[s] = bf16x8_from_bits(u.x, u.y, u.z, u.w);
+        }
+    } else if constexpr (VAR == 1) {
+        const uint4 *src = reinterpret_cast<const uint4 *>(L.tiles) + (rb * kF8Steps * 64 + lane) * 2;
+#pragma unroll
+        for (int s = 0; s < kF8Steps; ++s) {
+            const uint4 u0 = src[s * 128], u1 = src[s * 128 + 1];
+            i32x8 w = {(int)u0.x, (int)u0.y, (int)u0.z, (int)u0.w, (int)u1.x, (int)u1.y, (int)u1.z, (int)u1.w};
+            r.v[s] = w;
+        }
+        r.scale = (int)L.scale[t + (lane & 31)] * 0x01010101;
+    } else {
+        const uint2 *src = reinterpret_cast<const uint2 *>(L.tiles) + rb * kBfSteps * 64 + lane;
+#pragma unroll
+        for (int s = 0; s < kBfSteps; ++s) {
+            const uint2 u = src[s * 64];
+            // uint8 codes -> bf16 (exact): (float)byte compiles to
+            // v_cvt_f32_ubyteN; the top half of the float is the bf16.
+            const float c0 = (float)(u.x & 0xFFu), c1 = (float)((u.x >> 8) & 0xFFu);
+            const float c2 = (float)((u.x >> 16) & 0xFFu), c3 = (float)(u.x >> 24);
+            const float c4 = (float)(u.y & 0xFFu), c5 = (float)((u.y >> 8) & 0xFFu);
+            const float c6 = (float)((u.y >> 16) & 0xFFu), c7 = (float)(u.y >> 24);
+            r.v[s] = bf16x8_from_bits(pack_bf16_exact(c0, c1), pack_bf16_exact(c2, c3), pack_bf16_exact(c4, c5),
+                                      pack_bf16_exact(c6, c7));
+        }
+    }
+}
+
+// Query fragment of this lane's column (query qi of the batch), or zeros.
+template <int VAR>
+__device__ inline void load_query(const ScanMfmaArgs &a, bool valid, uint32_t qi, uint32_t lane, QFrag<VAR> &f) {
+    const uint32_t half = lane >> 5;
+    const float *q = a.queries + (uint64_t)qi * kDim;
+    if constexpr (VAR == 0 || VAR == 2) {
+#pragma unroll
+        for (int s = 0; s < kBfSteps; ++s) {
+            uint32_t w[4];
+#pragma unroll
+            for (int e2 = 0; e2 < 4; ++e2) {
+                uint16_t h[2];
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const int k = 16 * s + 8 * half + 2 * e2 + u;
+                    float v = 0.f;
+                    if (valid) {
+                        if (k < kDim) v = VAR == 0 ? -2.0f * q[k] : -2.0f * q[k] * a.sq8_w[k];
+                        else if (VAR == 0 && (k == kDim || k == kDim + 1)) v = 1.0f;
+                    }
+                    h[u] = bf16_bits(v);
+                }
+                w[e2] = h[0] | (uint32_t)h[1] << 16;
+            }
+            f.v[s] = bf16x8_from_bits(w[0], w[1], w[2], w[3]);
+        }
+    } else {
+        float am = 0.f;
+        if (valid)
+            for (int k = 0; k < kDim; ++k) am = fmaxf(am, fabsf(q[k]));
+        const int ex = fp8_exponent(am);
+#pragma unroll
+        for (int s = 0; s < kF8Steps; ++s) {
+            i32x8 w;
+#pragma unroll
+            for (int g = 0; g < 8; ++g) {
+                float v[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int k = 64 * s + 32 * half + 4 * g + u;
+                    v[u] = (valid && k < kDim) ? -ldexpf(q[k], -ex) : 0.f;
+                }
+                w[g] = (int)pack_fp8x4(v[0], v[1], v[2], v[3]);
+            }
+            f.v[s] = w;
+        }
+        // B values carry -q / 2^ex; the scale 2^(ex+1) restores -2 q.
+        f.scale = (127 + ex + 1) * 0x01010101;
+    }
+}
+
+template <int VAR>
+__device__ inline f32x16 tile_mma(const RowFrag<VAR> &r, const QFrag<VAR> &q) {
+    f32x16 acc = {};
+    if constexpr (VAR == 1) {
+#pragma unroll
+        for (int s = 0; s < kF8Steps; ++s)
+            acc = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(r.v[s], q.v[s], acc, 0, 0, 0, r.scale, 0, q.scale);
+    } else {
+#pragma unroll
+        for (int s = 0; s < kBfSteps; ++s) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(r.v[s], q.v[s], acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+__device__ inline float tau_as_float(uint64_t tau) {
+    return tau == kKeyMax ? __builtin_inff() : ord2f((uint32_t)(tau >> 32));
+}
+
+template <int VAR, int QB, int B>
+__device__ void scan_mfma_item(const ScanMfmaArgs &a, const WorkItem &W, uint64_t *sbuf, uint64_t *s_tau,
+                               uint32_t *s_cnt) {
+    const uint32_t lane = threadIdx.x, col = lane & 31, half = lane >> 5;
+    const uint32_t nq = W.q_count;
+    const LayoutMfma &L = a.lay[W.layout];
+    for (uint32_t j = lane; j < nq; j += 64) {
+        s_tau[j] = kKeyMax;
+        s_cnt[j] = 0;
+    }
+    QFrag<VAR> qf[QB];
+    float tauf[QB];
+    uint32_t qs[QB], qe[QB];
+    bool qv[QB];
+#pragma unroll
+    for (int b = 0; b < QB; ++b) {
+        const uint32_t j = 32 * b + col;
+        qv[b] = j < nq;
+        uint32_t qi = 0;
+        qs[b] = qe[b] = 0;
+        if (qv[b]) {
+            const ItemQuery e = a.iq[W.q_begin + j];
+            qi = e.qi;
+            qs[b] = e.s;
+            qe[b] = e.e;
+        }
+        load_query<VAR>(a, qv[b], qi, lane, qf[b]);
+        tauf[b] = qv[b] ? __builtin_inff() : -__builtin_inff();
+    }
+    const uint32_t nblk = (nq + 31) / 32;
+    __syncthreads();
+
+#pragma unroll 1
+    for (uint32_t t = W.row_begin; t < W.row_end; t += kMTile) {
+        uint64_t hit[QB];
+        bool any = false;
+#pragma unroll
+        for (int b = 0; b < QB; ++b) {
+            hit[b] = __ballot(qv[b] && t < qe[b] && t + kMTile > qs[b]);
+            any |= hit[b] != 0;
+        }
+        if (!any) continue;
+        RowFrag<VAR> rf;
+        load_rows<VAR>(a, L, t, lane, rf);
+        float xn[16];
+        if constexpr (VAR != 0) {
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const float4 v = *reinterpret_cast<const float4 *>(L.xnorm + t + 8 * g + 4 * half);
+                xn[4 * g + 0] = v.x; xn[4 * g + 1] = v.y; xn[4 * g + 2] = v.z; xn[4 * g + 3] = v.w;
+            }
+        }
+        bool appended = false;
+#pragma unroll
+        for (int b = 0; b < QB; ++b) {
+            if ((uint32_t)b >= nblk || hit[b] == 0) continue;
+            f32x16 acc = tile_mma<VAR>(rf, qf[b]);
+            float sc[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) sc[j] = VAR == 0 ? acc[j] : acc[j] + xn[j];
+            uint64_t anym = 0;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) anym |= __ballot(sc[j] <= tauf[b]);
+            const bool partial = __ballot(qv[b] && !(t >= qs[b] && t + kMTile <= qe[b])) != 0;
+            if (anym == 0) continue;
+            // Rare path: exact key test and append.
+            appended = true;
+            const uint32_t qj = 32 * b + col;
+            uint64_t *list = a.lists + (W.slot_begin + qj) * (uint64_t)B;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const uint32_t row = t + 8 * (j / 4) + 4 * half + (j % 4);
+                bool ok = qv[b] && sc[j] <= tauf[b];
+                if (partial) ok = ok && row >= qs[b] && row < qe[b];
+                if (ok) {
+                    const uint64_t key = ((uint64_t)f2ord(sc[j]) << 32) | L.ids[row];
+                    if (key < s_tau[qj]) {
+                        const uint32_t pos = atomicAdd(&s_cnt[qj], 1u);
+                        list[pos] = key;
+                    }
+                }
+            }
+        }
+        if (!appended) continue;
+        __syncthreads();
+        // Compact every list that could overflow on the next tile.
+        bool changed = false;
+#pragma unroll 1
+        for (uint32_t j0 = 0; j0 < nq; j0 += 64) {
+            const uint32_t j = j0 + lane;
+            uint64_t need = __ballot(j < nq && s_cnt[j] > (uint32_t)(B - kMTile));
+            while (need) {
+                const uint32_t jj = j0 + (uint32_t)__builtin_ctzll(need);
+                need &= need - 1;
+                uint32_t cnt = s_cnt[jj];
+                uint64_t tau = s_tau[jj];
+                compact_global<B>(a.lists + (W.slot_begin + jj) * (uint64_t)B, cnt, tau, a.K, sbuf);
+                if (lane == 0) {
+                    s_cnt[jj] = cnt;
+                    s_tau[jj] = tau;
+                }
+                changed = true;
+                __syncthreads();
+            }
+        }
+        if (changed) {
+#pragma unroll
+            for (int b = 0; b < QB; ++b)
+                if (qv[b]) tauf[b] = tau_as_float(s_tau[32 * b + col]);
+        }
+    }
+    __syncthreads();
+    for (uint32_t j = 0; j < nq; ++j) {
+        uint32_t cnt = s_cnt[j];
+        uint64_t tau = s_tau[j];
+        compact_global<B>(a.lists + (W.slot_begin + j) * (uint64_t)B, cnt, tau, a.K, sbuf);
+        if (lane == 0) a.counts[W.slot_begin + j] = cnt;
+    }
+}
+
+template <int VAR, int QB, int B>
+__global__ __launch_bounds__(64) void scan_mfma_kernel(ScanMfmaArgs a) {
+    __shared__ uint64_t sbuf[B];
+    __shared__ uint64_t s_tau[32 * QB];
+    __shared__ uint32_t s_cnt[32 * QB];
+    if (a.queue == nullptr) {
+        const WorkItem W = as_const(a.items)[blockIdx.x];
+        scan_mfma_item<VAR, QB, B>(a, W, sbuf, s_tau, s_cnt);
+        return;
+    }
+    for (;;) {
+        uint32_t idx = 0;
+        if (threadIdx.x == 0) idx = atomicAdd(a.queue, 1u);
+        idx = __builtin_amdgcn_readfirstlane(__shfl(idx, 0));
+        if (idx >= a.n_items) break;
+        const WorkItem W = as_const(a.items)[idx];
+        scan_mfma_item<VAR, QB, B>(a, W, sbuf, s_tau, s_cnt);
+        __syncthreads();
+    }
+}
+
+template <int VAR, int QB>
+static hipError_t launch_mfma_qb(const ScanMfmaArgs &a, int B, uint64_t grid, hipStream_t st) {
+    switch (B) {
+    case 256: hipLaunchKernelGGL((scan_mfma_kernel<VAR, QB, 256>), dim3((uint32_t)grid), dim3(64), 0, st, a); break;
+    case 512: hipLaunchKernelGGL((scan_mfma_kernel<VAR, QB, 512>), dim3((uint32_t)grid), dim3(64), 0, st, a); break;
+    case 1024: hipLaunchKernelGGL((scan_mfma_kernel<VAR, QB, 1024>), dim3((uint32_t)grid), dim3(64), 0, st, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_scan_mfma(int variant, const ScanMfmaArgs &a, int B, int qb, hipStream_t st) {
+    if (a.n_items == 0) return hipSuccess;
+    uint64_t grid = a.n_items;
+    if (a.queue) {
+        int dev = 0, cus = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        grid = std::min<uint64_t>((uint64_t)cus * 8, a.n_items);
+    }
+    if (qb == 1) {
+        if (variant == 0) return launch_mfma_qb<0, 1>(a, B, grid, st);
+        if (variant == 1) return launch_mfma_qb<1, 1>(a, B, grid, st);
+        return launch_mfma_qb<2, 1>(a, B, grid, st);
+    }
+    if (variant == 0) return launch_mfma_qb<0, 2>(a, B, grid, st);
+    if (variant == 1) return launch_mfma_qb<1, 2>(a, B, grid, st);
+    return launch_mfma_qb<2, 2>(a, B, grid, st);
+}
+
+}  // namespace fbf

@@ -146,3 +146,63 @@ hipError_t launch_scan_sq8sym(const ScanSymArgs &a, int B, hipStream_t st) {
 }
 
 }  // namespace fbf
+
+namespace fbf {
+
+// ---------------------------------------------------------------- symmetric int8 encoding (device)
+
+// Global absmax (SQ8SymmetricQuantizer::train, sq8_sym_quant.hpp:37-43):
+// non-negative floats order like their bit patterns, so an unsigned atomicMax
+// on the bits reduces them exactly.
+__global__ void absmax_kernel(const float *__restrict__ x, uint64_t count, uint32_t *__restrict__ out_bits) {
+    uint32_t m = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        m = max(m, __builtin_bit_cast(uint32_t, fabsf(x[i])));
+    for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(out_bits, m);
+}
+
+// SQ8SymmetricQuantizer::encode (sq8_sym_quant.hpp:63-74): round(clamp(x /
+// alpha, -1, 1) * 127), half away from zero, 112-byte codes zero padded.
+// Row p of the output takes source row perm[p] (perm may be null: identity);
+// rows at or beyond n are all zero.
+__global__ void sym_encode_kernel(const float *__restrict__ rows, const uint32_t *__restrict__ perm,
+                                  uint64_t n, uint64_t n_out, float alpha, int4 *__restrict__ out) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_out * kCodeInt4) return;
+    const uint64_t p = t / kCodeInt4;
+    const int c = (int)(t % kCodeInt4);
+    uint32_t w[4] = {0, 0, 0, 0};
+    if (p < n) {
+        const float *x = rows + (uint64_t)(perm ? perm[p] : p) * kDim;
+        for (int e = 0; e < 16; ++e) {
+            const int k = 16 * c + e;
+            int8_t code = 0;
+            if (k < kDim) {
+                float v = x[k] / alpha;
+                if (v > 1.0f) v = 1.0f;
+                if (v < -1.0f) v = -1.0f;
+                code = (int8_t)roundf(v * 127.0f);
+            }
+            w[e / 4] |= (uint32_t)(uint8_t)code << (8 * (e % 4));
+        }
+    }
+    out[t] = make_int4((int)w[0], (int)w[1], (int)w[2], (int)w[3]);
+}
+
+hipError_t launch_absmax(const float *x, uint64_t count, uint32_t *out_bits, hipStream_t st) {
+    hipLaunchKernelGGL(absmax_kernel, dim3(1024), dim3(256), 0, st, x, count, out_bits);
+    return hipGetLastError();
+}
+
+hipError_t launch_sym_encode(const float *rows, const uint32_t *perm, uint64_t n, uint64_t n_out, float alpha,
+                             int8_t *out, hipStream_t st) {
+    const uint64_t total = n_out * kCodeInt4;
+    if (total == 0) return hipSuccess;
+    hipLaunchKernelGGL(sym_encode_kernel, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, st, rows, perm, n,
+                       n_out, alpha, reinterpret_cast<int4 *>(out));
+    return hipGetLastError();
+}
+
+}  // namespace fbf

@@ -1,0 +1,121 @@
+"""Approximate modes (BF16 / FP8 / SQ8 MFMA scans, symmetric int8 scan) with
+the exact FP32 rerank, through the C-ABI, against the CPU oracle.
+
+Invariants checked for every mode:
+* every returned (id, distance) is a row that passes the query's filter and
+  its distance is the canonical FP32 distance of that row (bit-exact: the
+  rerank recomputes it the oracle's way);
+* the list is ascending by (distance, id) and padded only when fewer than k
+  rows pass the filter;
+* when a query admits at most k' rows, the rerank sees all of them and the
+  result equals the exact search bit for bit;
+* recall@100 against the exact search meets the mode's floor.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+MODES = ["bf16", "sq8", "fp8", "sq8sym"]
+RECALL_FLOOR = {"bf16": 0.99, "sq8": 0.99, "fp8": 0.95, "sq8sym": 0.99}
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def c0(pkg):
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    dg = pkg.datagen
+    cfg = dg.CONFIGS[0]
+    vecs, labels, ts = dg.make_base(cfg)
+    q, metas = dg.make_queries(cfg, nq=400)
+    fbf = pkg.fbf
+    ds = fbf.Dataset(vecs, labels, ts, repr_flags=fbf.REPR_ALL)
+    yield vecs, labels, ts, q, metas, ds
+    ds.close()
+
+
+def _search(ds, q, metas, k, mode, **kw):
+    import torch
+    qd = torch.from_numpy(np.ascontiguousarray(q)).cuda()
+    ids, d = ds.search(qd, metas, k=k, mode=mode, **kw)
+    torch.cuda.synchronize()
+    return ids.cpu().numpy().view(np.uint32), d.cpu().numpy()
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_mode_invariants_and_recall(pkg, orc, c0, mode):
+    _say(f"case: config0 N=10000 nq=400 mode={mode} k=100 k'=400")
+    vecs, labels, ts, q, metas, ds = c0
+    lab = labels.astype(np.int32)
+    ids, d = _search(ds, q, metas, 100, mode)
+    eid, ed = orc.exact_predicate(vecs, lab, ts, q, metas, k=100, mode="canon")
+    L = orc.Layouts(lab, ts)
+    for i in range(q.shape[0]):
+        lay, s, e = L.query_range(*metas[i])
+        cnt = e - s
+        valid = ids[i] != 0xFFFFFFFF
+        assert valid.sum() == min(100, cnt)
+        assert np.all(valid[: valid.sum()])
+        for j in np.nonzero(valid)[0]:
+            r = int(ids[i, j])
+            assert orc.match(*metas[i], lab[r], ts[r])
+            assert d[i, j] == np.float32(orc.l2_canon(vecs[r], q[i]))
+        keys = orc.make_keys(d[i][valid], ids[i][valid])
+        assert np.all(np.diff(keys.astype(np.float64)) > 0) or valid.sum() <= 1
+        if cnt <= 400:
+            assert np.array_equal(ids[i], eid[i]), (mode, i, cnt)
+            assert np.array_equal(d[i], ed[i].astype(np.float32))
+    rec, _ = orc.recall(ids, eid)
+    _say(f"  recall@100 vs exact = {rec:.4f}")
+    assert rec >= RECALL_FLOOR[mode]
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_mode_small_kprime_and_chunks(pkg, orc, c0, mode):
+    _say(f"case: mode={mode} k=10 k'=40 chunk=128 group=7 persistent")
+    vecs, labels, ts, q, metas, ds = c0
+    ids, d = _search(ds, q[:64], metas[:64], 10, mode, kprime=40, chunk_rows=128, group_queries=7,
+                     persistent=True)
+    eid, ed = orc.exact_predicate(vecs, labels.astype(np.int32), ts, q[:64], metas[:64], k=10, mode="canon")
+    rec, _ = orc.recall(ids, eid)
+    assert rec >= RECALL_FLOOR[mode] - 0.05
+    for i in range(64):
+        valid = ids[i] != 0xFFFFFFFF
+        for j in np.nonzero(valid)[0]:
+            assert d[i, j] == np.float32(orc.l2_canon(vecs[int(ids[i, j])], q[i]))
+
+
+def test_modes_small_filters_equal_exact(pkg, orc):
+    """With N small every query admits at most k' rows: all modes must return
+    exactly the exact search (ties to the smaller id included)."""
+    import torch
+    _say("case: N=350 all modes == exact")
+    fbf, dg = pkg.fbf, pkg.datagen
+    cfg = dg.CONFIGS[0]
+    vecs, labels, ts = dg.make_base(cfg, n=350)
+    vecs[100:120] = vecs[5]  # exact ties
+    q, metas = dg.make_queries(cfg, nq=60)
+    ds = fbf.Dataset(vecs, labels, ts, repr_flags=fbf.REPR_ALL)
+    try:
+        eid, ed = orc.exact_predicate(vecs, labels.astype(np.int32), ts, q, metas, k=100, mode="canon")
+        for mode in MODES:
+            ids, d = _search(ds, q, metas, 100, mode)
+            assert np.array_equal(ids, eid), mode
+            assert np.array_equal(d, ed.astype(np.float32)), mode
+    finally:
+        ds.close()
+
+
+def test_quant_params(pkg, orc, c0):
+    vecs, labels, ts, q, metas, ds = c0
+    alpha, mi, dif = ds.quant_params()
+    assert alpha == orc.sq8sym_train(vecs)
+    mi2, dif2 = orc.sq8_train(vecs)
+    assert np.array_equal(mi, mi2) and np.array_equal(dif, dif2)
