@@ -106,11 +106,11 @@ struct ProfEvent {
 // Per-call scratch: the uploaded plan, the per-(item, query) candidate lists
 // and the pinned staging buffer the plan is copied through.
 struct Scratch {
-    DevBuf d_items, d_iq, d_slot_off, d_qslots, d_lists, d_counts, d_queue, d_q, d_out_ids, d_out_d;
+    DevBuf d_items, d_iq, d_slot_off, d_qslots, d_lists, d_counts, d_queue, d_q, d_out_ids, d_out_d, d_gtau;
     HostPinned h_stage;
     void release() {
         for (DevBuf *b : {&d_items, &d_iq, &d_slot_off, &d_qslots, &d_lists, &d_counts, &d_queue,
-                          &d_q, &d_out_ids, &d_out_d})
+                          &d_q, &d_out_ids, &d_out_d, &d_gtau})
             b->release();
         h_stage.release();
     }
@@ -597,6 +597,10 @@ extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, cons
     if (s != FBF_OK) return s;
     uint32_t *queue = nullptr;
     if (o.persistent) queue = S.d_queue.as<uint32_t>();
+    // Per-query thresholds shared by all work items of a query.
+    HIP_TRY(S.d_gtau.reserve(nq * sizeof(uint64_t)));
+    uint64_t *gtau = S.d_gtau.as<uint64_t>();
+    HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(gtau), 0xFFFFFFFFu, 2 * nq, st));
 
     // Scan.  Each work item writes every list slot it owns (count and keys),
     // so no clearing is needed.
@@ -617,6 +621,7 @@ extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, cons
         }
         a.lists = S.d_lists.as<uint64_t>();
         a.counts = S.d_counts.as<uint32_t>();
+        a.gtau = gtau;
         a.K = K;
         a.queue = queue;
         if (queue) HIP_TRY(launch_fill_u32(queue, 0u, 1, st));
@@ -649,6 +654,7 @@ extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, cons
                 a.ids = ds->lay[L].ids;
                 a.lists = S.d_lists.as<uint64_t>();
                 a.counts = S.d_counts.as<uint32_t>();
+                a.gtau = gtau;
                 a.K = K;
                 a.queue = queue;
                 HIP_TRY(launch_scan_f32(a, B, st));

@@ -79,6 +79,7 @@ struct ScanArgs {
     const uint32_t *ids;       // layout position -> original id
     uint64_t *lists;           // [slots][B]
     uint32_t *counts;          // [slots]
+    uint64_t *gtau;            // [nq] shared per-query threshold (key)
     uint32_t K;
     uint32_t *queue;           // persistent mode work counter (or null)
 };
@@ -117,6 +118,7 @@ struct ScanMfmaArgs {
     LayoutMfma lay[2];
     uint64_t *lists;
     uint32_t *counts;
+    uint64_t *gtau;          // [nq] shared per-query threshold (key), tightened by every item
     uint32_t K;
     uint32_t *queue;
 };

@@ -95,8 +95,13 @@ __device__ void scan_f32_item(const ScanArgs &a, const WorkItem &W, uint64_t *sb
     }
     __syncthreads();
 
+    uint32_t since_share = 0;
 #pragma unroll 1
     for (uint32_t t = W.row_begin; t < W.row_end; t += kRowTile) {
+        if (++since_share == 32) {
+            since_share = 0;
+            share_thresholds(a.gtau, nq, s_tau, [&](uint32_t j) { return iq[j].qi; });
+        }
         const uint32_t row = t + lane;
         float x[kDim];
         load_row(soa, t, lane, x);
@@ -137,7 +142,9 @@ __device__ void scan_f32_item(const ScanArgs &a, const WorkItem &W, uint64_t *sb
         uint64_t *list = a.lists + (W.slot_begin + j) * (uint64_t)B;
         compact_global<B>(list, cnt, tau, a.K, sbuf);
         if (lane == 0) a.counts[W.slot_begin + j] = cnt;
+        s_tau[j] = tau;
     }
+    share_thresholds(a.gtau, nq, s_tau, [&](uint32_t j) { return iq[j].qi; });
 }
 
 template <int B>

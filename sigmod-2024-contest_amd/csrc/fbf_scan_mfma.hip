@@ -17,7 +17,8 @@
 //         scale operands; score = acc + |x|^2 (FP32, epilogue).
 //   SQ8:  per-dimension uint8 codes (SQ8Quantizer, sq8_quant.hpp:30-63)
 //         dequantised in registers to bf16 (codes 0..255 are exact in bf16);
-//         the query side carries dif_i / 255; score = acc + |xhat|^2.
+//         the query side carries dif_i / 255; |xhat|^2 rides in the spare K
+//         slots as for BF16: score = acc.
 //
 // MFMA lane layouts (measured on gfx950):
 //   v_mfma_f32_32x32x16_bf16: A lane l holds row l%32, k = 8*(l/32) + e;
@@ -122,8 +123,12 @@ __global__ void build_fp8_kernel(const float *__restrict__ rows, const uint32_t 
 
 // SQ8 codes in MFMA order: block rb, step s, lane l: 8 bytes = codes of row
 // rb*32 + l%32 at k = 16 s + 8 (l/32) + e (zero beyond d).
+// Bytes 4..7 of step 6, lanes 0..31 (k = 100..103, beyond the 100 codes)
+// carry the bf16 hi / lo split of |xhat|^2; the scan passes them through
+// unconverted, so the norm rides in the MFMA like the BF16 variant's.
 __global__ void build_sq8_kernel(const uint8_t *__restrict__ codes_orig, const uint32_t *__restrict__ perm,
-                                 uint64_t n, uint64_t n_blocks, uint2 *__restrict__ out) {
+                                 const float *__restrict__ norms, uint64_t n, uint64_t n_blocks,
+                                 uint2 *__restrict__ out) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n_blocks * kBfSteps * 64) return;
     const uint32_t l = t % 64, s = (t / 64) % kBfSteps;
@@ -136,6 +141,11 @@ __global__ void build_sq8_kernel(const uint8_t *__restrict__ codes_orig, const u
             const int k = 16 * s + 8 * (l / 32) + e;
             const uint32_t v = k < kDim ? c[k] : 0u;
             if (e < 4) lo |= v << (8 * e); else hi |= v << (8 * (e - 4));
+        }
+        if (s == kBfSteps - 1 && l < 32) {
+            const float nn = norms[perm[p]];
+            const uint16_t nh = bf16_bits(nn), nl = bf16_bits(nn - bf16_to_f(nh));
+            hi = nh | (uint32_t)nl << 16;
         }
     }
     out[t] = make_uint2(lo, hi);
@@ -232,7 +242,7 @@ hipError_t launch_build_mfma(int variant, const float *rows, const uint32_t *per
     } else {
         const uint64_t total = n_blocks * kBfSteps * 64;
         hipLaunchKernelGGL(build_sq8_kernel, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, st,
-                           sq8_codes, perm, n, n_blocks, (uint2 *)out);
+                           sq8_codes, perm, norms, n, n_blocks, (uint2 *)out);
     }
     return hipGetLastError();
 }
@@ -319,8 +329,12 @@ __device__ inline void load_rows(const ScanMfmaArgs &a, const LayoutMfma &L, uin
             const float c2 = (float)((u.x >> 16) & 0xFFu), c3 = (float)(u.x >> 24);
             const float c4 = (float)(u.y & 0xFFu), c5 = (float)((u.y >> 8) & 0xFFu);
             const float c6 = (float)((u.y >> 16) & 0xFFu), c7 = (float)(u.y >> 24);
-            r.v[s] = bf16x8_from_bits(pack_bf16_exact(c0, c1), pack_bf16_exact(c2, c3), pack_bf16_exact(c4, c5),
-                                      pack_bf16_exact(c6, c7));
+            uint32_t w2 = pack_bf16_exact(c4, c5), w3 = pack_bf16_exact(c6, c7);
+            if (s == kBfSteps - 1 && lane < 32) {  // k = 100..103: bf16 norm split, not codes
+                w2 = u.y;
+                w3 = 0u;
+            }
+            r.v[s] = bf16x8_from_bits(pack_bf16_exact(c0, c1), pack_bf16_exact(c2, c3), w2, w3);
         }
     }
 }
@@ -343,7 +357,7 @@ __device__ inline void load_query(const ScanMfmaArgs &a, bool valid, uint32_t qi
                     float v = 0.f;
                     if (valid) {
                         if (k < kDim) v = VAR == 0 ? -2.0f * q[k] : -2.0f * q[k] * a.sq8_w[k];
-                        else if (VAR == 0 && (k == kDim || k == kDim + 1)) v = 1.0f;
+                        else if (k == kDim || k == kDim + 1) v = 1.0f;  // |x|^2 slots
                     }
                     h[u] = bf16_bits(v);
                 }
@@ -426,41 +440,51 @@ __device__ void scan_mfma_item(const ScanMfmaArgs &a, const WorkItem &W, uint64_
     const uint32_t nblk = (nq + 31) / 32;
     __syncthreads();
 
+    // Rows stream through a register double buffer: tile t + 32 is loaded
+    // while tile t is on the matrix cores.
+    RowFrag<VAR> rf;
+    float xn[16];
+    if (W.row_begin < W.row_end) load_rows<VAR>(a, L, W.row_begin, lane, rf);
+    if constexpr (VAR == 1) {
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const float4 v = *reinterpret_cast<const float4 *>(L.xnorm + W.row_begin + 8 * g + 4 * half);
+            xn[4 * g + 0] = v.x; xn[4 * g + 1] = v.y; xn[4 * g + 2] = v.z; xn[4 * g + 3] = v.w;
+        }
+    }
+    uint32_t since_sync = 0;
 #pragma unroll 1
     for (uint32_t t = W.row_begin; t < W.row_end; t += kMTile) {
-        uint64_t hit[QB];
-        bool any = false;
+        RowFrag<VAR> nf;
+        float nxn[16];
+        const uint32_t tn = t + kMTile;
+        if (tn < W.row_end) {
+            load_rows<VAR>(a, L, tn, lane, nf);
+            if constexpr (VAR == 1) {
 #pragma unroll
-        for (int b = 0; b < QB; ++b) {
-            hit[b] = __ballot(qv[b] && t < qe[b] && t + kMTile > qs[b]);
-            any |= hit[b] != 0;
-        }
-        if (!any) continue;
-        RowFrag<VAR> rf;
-        load_rows<VAR>(a, L, t, lane, rf);
-        float xn[16];
-        if constexpr (VAR != 0) {
-#pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                const float4 v = *reinterpret_cast<const float4 *>(L.xnorm + t + 8 * g + 4 * half);
-                xn[4 * g + 0] = v.x; xn[4 * g + 1] = v.y; xn[4 * g + 2] = v.z; xn[4 * g + 3] = v.w;
+                for (int g = 0; g < 4; ++g) {
+                    const float4 v = *reinterpret_cast<const float4 *>(L.xnorm + tn + 8 * g + 4 * half);
+                    nxn[4 * g + 0] = v.x; nxn[4 * g + 1] = v.y; nxn[4 * g + 2] = v.z; nxn[4 * g + 3] = v.w;
+                }
             }
         }
         bool appended = false;
 #pragma unroll
         for (int b = 0; b < QB; ++b) {
-            if ((uint32_t)b >= nblk || hit[b] == 0) continue;
+            if ((uint32_t)b >= nblk) continue;
+            if (__ballot(qv[b] && t < qe[b] && t + kMTile > qs[b]) == 0) continue;
             f32x16 acc = tile_mma<VAR>(rf, qf[b]);
             float sc[16];
 #pragma unroll
-            for (int j = 0; j < 16; ++j) sc[j] = VAR == 0 ? acc[j] : acc[j] + xn[j];
-            uint64_t anym = 0;
+            for (int j = 0; j < 16; ++j) sc[j] = VAR == 1 ? acc[j] + xn[j] : acc[j];
+            // One compare per lane: the smallest of its 16 scores.
+            float mn = sc[0];
 #pragma unroll
-            for (int j = 0; j < 16; ++j) anym |= __ballot(sc[j] <= tauf[b]);
-            const bool partial = __ballot(qv[b] && !(t >= qs[b] && t + kMTile <= qe[b])) != 0;
-            if (anym == 0) continue;
+            for (int j = 1; j < 16; ++j) mn = fminf(mn, sc[j]);
+            if (__ballot(mn <= tauf[b]) == 0) continue;
             // Rare path: exact key test and append.
             appended = true;
+            const bool partial = __ballot(qv[b] && !(t >= qs[b] && t + kMTile <= qe[b])) != 0;
             const uint32_t qj = 32 * b + col;
             uint64_t *list = a.lists + (W.slot_begin + qj) * (uint64_t)B;
 #pragma unroll
@@ -477,9 +501,16 @@ __device__ void scan_mfma_item(const ScanMfmaArgs &a, const WorkItem &W, uint64_
                 }
             }
         }
-        if (!appended) continue;
+        rf = nf;
+        if constexpr (VAR == 1) {
+#pragma unroll
+            for (int j = 0; j < 16; ++j) xn[j] = nxn[j];
+        }
+        ++since_sync;
+        if (!appended && since_sync < 64) continue;
         __syncthreads();
-        // Compact every list that could overflow on the next tile.
+        // Compact every list that could overflow on the next tile, and pull in
+        // thresholds other items of the same queries have published.
         bool changed = false;
 #pragma unroll 1
         for (uint32_t j0 = 0; j0 < nq; j0 += 64) {
@@ -499,6 +530,11 @@ __device__ void scan_mfma_item(const ScanMfmaArgs &a, const WorkItem &W, uint64_
                 __syncthreads();
             }
         }
+        if (since_sync >= 64) {
+            since_sync = 0;
+            share_thresholds(a.gtau, nq, s_tau, [&](uint32_t j) { return a.iq[W.q_begin + j].qi; });
+            changed = true;
+        }
         if (changed) {
 #pragma unroll
             for (int b = 0; b < QB; ++b)
@@ -510,12 +546,18 @@ __device__ void scan_mfma_item(const ScanMfmaArgs &a, const WorkItem &W, uint64_
         uint32_t cnt = s_cnt[j];
         uint64_t tau = s_tau[j];
         compact_global<B>(a.lists + (W.slot_begin + j) * (uint64_t)B, cnt, tau, a.K, sbuf);
-        if (lane == 0) a.counts[W.slot_begin + j] = cnt;
+        if (lane == 0) {
+            a.counts[W.slot_begin + j] = cnt;
+            if (tau != kKeyMax) atomicMin((unsigned long long *)&a.gtau[a.iq[W.q_begin + j].qi], tau);
+        }
     }
 }
 
+// At least two waves per SIMD (<= 256 registers) so one wave's loads and
+// selection overlap another's matrix work.
 template <int VAR, int QB, int B>
-__global__ __launch_bounds__(64) void scan_mfma_kernel(ScanMfmaArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(QB == 1 ? 3 : 2, 8)))
+void scan_mfma_kernel(ScanMfmaArgs a) {
     __shared__ uint64_t sbuf[B];
     __shared__ uint64_t s_tau[32 * QB];
     __shared__ uint32_t s_cnt[32 * QB];

@@ -40,7 +40,7 @@ __device__ inline void wave_sort_lds(uint64_t *s) {
     for (int size = 2; size <= B; size <<= 1) {
 #pragma unroll 1
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
-#pragma unroll
+#pragma unroll 2
             for (int p0 = 0; p0 < B / 2; p0 += 64) {
                 const int p = p0 + (int)lane;
                 const int i = 2 * p - (p & (stride - 1));
@@ -120,6 +120,23 @@ __device__ inline void offer_lds(uint64_t *list, uint32_t &cnt, uint64_t &tau, u
     if (pass) list[cnt + lanes_below(m)] = key;
     cnt += (uint32_t)__popcll(m);
     if (cnt > (uint32_t)(B - 64)) compact_lds<B>(list, cnt, tau, K);
+}
+
+// Publish this item's thresholds to the per-query shared thresholds and take
+// the tighter of the two (one 64-bit atomicMin per query both publishes and
+// fetches).  Valid because a key at or above the K-th smallest key of ANY
+// subset of a query's candidates cannot be in that query's top-K.  s_tau[j]
+// is the item's threshold of its query j, iq_qi(j) the query's batch index.
+template <class QiFn>
+__device__ inline void share_thresholds(uint64_t *gtau, uint32_t nq, uint64_t *s_tau, QiFn iq_qi) {
+    if (gtau == nullptr) return;
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < nq; j += 64) {
+        const uint64_t mine = s_tau[j];
+        const uint64_t old = atomicMin((unsigned long long *)&gtau[iq_qi(j)], (unsigned long long)mine);
+        s_tau[j] = old < mine ? old : mine;
+    }
+    __syncthreads();
 }
 
 }  // namespace fbf
