@@ -292,7 +292,8 @@ static fbf_status dataset_build(fbf_dataset *ds, const float *labels, const floa
         if (repr & FBF_REPR_SQ8) {
             HIP_TRY(hipMalloc(&D.sq8, nb32 * 7 * 64 * 8));
             HIP_TRY(hipMalloc(&D.sq8_norm, n_pad * sizeof(float)));
-            HIP_TRY(launch_build_mfma(2, nullptr, D.ids, nullptr, tmp_codes.as<uint8_t>(), n, D.sq8, nullptr, st));
+            HIP_TRY(launch_build_mfma(2, nullptr, D.ids, tmp_xhat.as<float>(), tmp_codes.as<uint8_t>(), n, D.sq8,
+                                      nullptr, st));
             HIP_TRY(launch_gather_f32(tmp_xhat.as<float>(), D.ids, n, n_pad, D.sq8_norm, st));
         }
         if (repr & FBF_REPR_SQ8SYM) {

@@ -231,6 +231,12 @@ hipError_t launch_build_mfma(int variant, const float *rows, const uint32_t *per
                              const uint8_t *sq8_codes, uint64_t n, void *out, uint8_t *scale,
                              hipStream_t st) {
     const uint64_t n_blocks = (n + kMTile - 1) / kMTile;
+    // Every input a variant reads must be present (BF16 and SQ8 fold the
+    // norms into their fragments; FP8 writes a scale per row).
+    if (perm == nullptr || out == nullptr) return hipErrorInvalidValue;
+    if (variant == 0 && (rows == nullptr || norms == nullptr)) return hipErrorInvalidValue;
+    if (variant == 1 && (rows == nullptr || scale == nullptr)) return hipErrorInvalidValue;
+    if (variant == 2 && (sq8_codes == nullptr || norms == nullptr)) return hipErrorInvalidValue;
     if (variant == 0) {
         const uint64_t total = n_blocks * kBfSteps * 64;
         hipLaunchKernelGGL(build_bf16_kernel, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, st, rows,
