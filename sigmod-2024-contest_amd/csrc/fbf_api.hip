@@ -536,14 +536,20 @@ static MergeArgs merge_args(const Scratch &S, int B, uint64_t nq, uint32_t k, ui
 }
 
 // ---------------------------------------------------------------- search
-// Work items in descending cost order (largest first), stable within a layout
-// run: big items start early and the small ones fill the tail.
+// Work items in descending cost order (largest first), grouped by layout
+// when each layout is its own launch: big items start early and the small ones
+// fill the tail.
 static void order_items(Plan &P, bool by_layout) {
     std::stable_sort(P.items.begin(), P.items.end(), [&](const WorkItem &x, const WorkItem &y) {
         if (by_layout && x.layout != y.layout) return x.layout < y.layout;
         const uint64_t cx = (uint64_t)(x.row_end - x.row_begin) * x.q_count;
         const uint64_t cy = (uint64_t)(y.row_end - y.row_begin) * y.q_count;
-        return cx > cy;
+        if (cx != cy) return cx > cy;
+        // Equal cost: chunk-major, so the items that run at the same time on an
+        // XCD (workgroups are dealt round-robin to the 8 XCDs) stream the same
+        // rows through that XCD's L2 for different query groups.
+        if (x.layout != y.layout) return x.layout < y.layout;
+        return x.row_begin < y.row_begin;
     });
 }
 
@@ -574,9 +580,9 @@ extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, cons
         if (K < k) return fail(FBF_ERR_INVALID_ARG, "kprime must be >= k");
         if (K > 896) return fail(FBF_ERR_INVALID_ARG, "kprime must be <= 896");
     }
-    const uint32_t chunk = o.chunk_rows ? (o.chunk_rows + kRowTile - 1) / kRowTile * kRowTile
-                                        : (mfma ? 1024u * 1024u : 256u * 1024u);
-    const uint32_t gcap = mfma ? 64u : (uint32_t)kMaxGroup;
+    const uint32_t chunk = o.chunk_rows ? (o.chunk_rows + kRowTile - 1) / kRowTile * kRowTile : 256u * 1024u;
+    // MFMA items carry up to 256 queries (4 waves x 64); VALU items up to 16.
+    const uint32_t gcap = mfma ? 256u : (uint32_t)kMaxGroup;
     const uint32_t gmax = o.group_queries ? std::min<uint32_t>(o.group_queries, gcap) : gcap;
     hipStream_t st = (hipStream_t)stream;
 
@@ -626,10 +632,10 @@ extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, cons
         a.K = K;
         a.queue = queue;
         if (queue) HIP_TRY(launch_fill_u32(queue, 0u, 1, st));
-        const int qb = gmax > 32 ? 2 : 1;
+        const int nw = gmax <= 64 ? 1 : gmax <= 128 ? 2 : 4;
         ProfEvent ev;
         record_begin(ds, 0, st, ev);
-        HIP_TRY(launch_scan_mfma(variant, a, B, qb, st));
+        HIP_TRY(launch_scan_mfma(variant, a, B, nw, st));
         record_end(ds, st, ev);
     } else {
         int8_t *qcodes = nullptr;

@@ -144,8 +144,9 @@ hipError_t launch_scan_f32(const ScanArgs &a, int B, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st);
 hipError_t launch_merge(const MergeArgs &a, hipStream_t st);
 hipError_t launch_scan_sq8sym(const ScanSymArgs &a, int B, hipStream_t st);
-// variant: 0 BF16, 1 FP8, 2 SQ8; qb: query blocks of 32 per work item (1 or 2)
-hipError_t launch_scan_mfma(int variant, const ScanMfmaArgs &a, int B, int qb, hipStream_t st);
+// variant: 0 BF16, 1 FP8, 2 SQ8; nw: waves per workgroup (1, 2, 4), an item
+// carries at most 64 * nw queries
+hipError_t launch_scan_mfma(int variant, const ScanMfmaArgs &a, int B, int nw, hipStream_t st);
 hipError_t launch_build_mfma(int variant, const float *rows, const uint32_t *perm, const float *norms,
                              const uint8_t *sq8_codes, uint64_t n, void *out, uint8_t *scale, hipStream_t st);
 hipError_t launch_row_norms(const float *rows, uint64_t n, float *out, hipStream_t st);

@@ -304,31 +304,34 @@ __device__ inline uint32_t pack_bf16_exact(float lo, float hi) {
     return (__builtin_bit_cast(uint32_t, lo) >> 16) | (__builtin_bit_cast(uint32_t, hi) & 0xFFFF0000u);
 }
 
+// Fragment bytes of one 32-row tile (contiguous in the layout's fragment
+// array): BF16 7 x 64 x 16 B, FP8 2 x 64 x 32 B, SQ8 7 x 64 x 8 B.
+template <int VAR> struct TileBytes;
+template <> struct TileBytes<0> { static constexpr int value = kBfSteps * 64 * 16; };
+template <> struct TileBytes<1> { static constexpr int value = kF8Steps * 64 * 32; };
+template <> struct TileBytes<2> { static constexpr int value = kBfSteps * 64 * 8; };
+
+// This lane's A fragments of a tile staged in LDS (layout as in HBM).
 template <int VAR>
-__device__ inline void load_rows(const ScanMfmaArgs &a, const LayoutMfma &L, uint32_t t, uint32_t lane,
-                                 RowFrag<VAR> &r) {
-    const uint64_t rb = t / kMTile;
+__device__ inline void frag_from_lds(const uint4 *sA, uint32_t lane, RowFrag<VAR> &r) {
     if constexpr (VAR == 0) {
-        const uint4 *src = reinterpret_cast<const uint4 *>(L.tiles) + rb * kBfSteps * 64 + lane;
 #pragma unroll
         for (int s = 0; s < kBfSteps; ++s) {
-            const uint4 u = src[s * 64];
+            const uint4 u = sA[s * 64 + lane];
             r.v[s] = bf16x8_from_bits(u.x, u.y, u.z, u.w);
         }
     } else if constexpr (VAR == 1) {
-        const uint4 *src = reinterpret_cast<const uint4 *>(L.tiles) + (rb * kF8Steps * 64 + lane) * 2;
 #pragma unroll
         for (int s = 0; s < kF8Steps; ++s) {
-            const uint4 u0 = src[s * 128], u1 = src[s * 128 + 1];
+            const uint4 u0 = sA[(s * 64 + lane) * 2], u1 = sA[(s * 64 + lane) * 2 + 1];
             i32x8 w = {(int)u0.x, (int)u0.y, (int)u0.z, (int)u0.w, (int)u1.x, (int)u1.y, (int)u1.z, (int)u1.w};
             r.v[s] = w;
         }
-        r.scale = (int)L.scale[t + (lane & 31)] * 0x01010101;
     } else {
-        const uint2 *src = reinterpret_cast<const uint2 *>(L.tiles) + rb * kBfSteps * 64 + lane;
+        const uint2 *src = reinterpret_cast<const uint2 *>(sA);
 #pragma unroll
         for (int s = 0; s < kBfSteps; ++s) {
-            const uint2 u = src[s * 64];
+            const uint2 u = src[s * 64 + lane];
             // uint8 codes -> bf16 (exact): (float)byte compiles to
             // v_cvt_f32_ubyteN; the top half of the float is the bf16.
             const float c0 = (float)(u.x & 0xFFu), c1 = (float)((u.x >> 8) & 0xFFu);
@@ -414,28 +417,45 @@ __device__ inline float tau_as_float(uint64_t tau) {
     return tau == kKeyMax ? __builtin_inff() : ord2f((uint32_t)(tau >> 32));
 }
 
-template <int VAR, int QB, int B>
-__device__ void scan_mfma_item(const ScanMfmaArgs &a, const WorkItem &W, uint64_t *sbuf, uint64_t *s_tau,
-                               uint32_t *s_cnt) {
-    const uint32_t lane = threadIdx.x, col = lane & 31, half = lane >> 5;
-    const uint32_t nq = W.q_count;
+// One workgroup of NW waves per work item.  The item's rows stream through a
+// double-buffered LDS tile (loaded once, read by every wave); wave w owns the
+// item's queries [64 w, 64 w + 64) as two 32-query B blocks in VGPRs, its own
+// candidate lists and thresholds, and synchronises with the other waves only
+// at the one barrier per tile that hands the next tile over.
+constexpr int kQB = 2;  // 32-query blocks per wave
+
+template <int VAR, int NW, int B>
+__device__ void scan_mfma_item(const ScanMfmaArgs &a, const WorkItem &W, uint4 (*sA)[TileBytes<VAR>::value / 16],
+                               uint64_t *sbuf, uint64_t *s_tau, uint32_t *s_cnt) {
+    constexpr int kU4 = TileBytes<VAR>::value / 16;
+    constexpr int kPer = (kU4 + 64 * NW - 1) / (64 * NW);  // staging loads per thread
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t col = lane & 31, half = lane >> 5;
+    const uint32_t nq_all = W.q_count;
+    const uint32_t q0 = 64 * wave;                            // this wave's first query of the item
+    const uint32_t nq = nq_all > q0 ? (nq_all - q0 < 64 ? nq_all - q0 : 64) : 0;
+    uint64_t *tau_w = s_tau + q0;
+    uint32_t *cnt_w = s_cnt + q0;
     const LayoutMfma &L = a.lay[W.layout];
+    const uint4 *gA = reinterpret_cast<const uint4 *>(L.tiles);
+    auto qi_of = [&](uint32_t j) { return a.iq[W.q_begin + q0 + j].qi; };
+
     for (uint32_t j = lane; j < nq; j += 64) {
-        s_tau[j] = kKeyMax;
-        s_cnt[j] = 0;
+        tau_w[j] = kKeyMax;
+        cnt_w[j] = 0;
     }
-    QFrag<VAR> qf[QB];
-    float tauf[QB];
-    uint32_t qs[QB], qe[QB];
-    bool qv[QB];
+    QFrag<VAR> qf[kQB];
+    float tauf[kQB];
+    uint32_t qs[kQB], qe[kQB];
+    bool qv[kQB];
 #pragma unroll
-    for (int b = 0; b < QB; ++b) {
+    for (int b = 0; b < kQB; ++b) {
         const uint32_t j = 32 * b + col;
         qv[b] = j < nq;
         uint32_t qi = 0;
         qs[b] = qe[b] = 0;
         if (qv[b]) {
-            const ItemQuery e = a.iq[W.q_begin + j];
+            const ItemQuery e = a.iq[W.q_begin + q0 + j];
             qi = e.qi;
             qs[b] = e.s;
             qe[b] = e.e;
@@ -443,174 +463,198 @@ __device__ void scan_mfma_item(const ScanMfmaArgs &a, const WorkItem &W, uint64_
         load_query<VAR>(a, qv[b], qi, lane, qf[b]);
         tauf[b] = qv[b] ? __builtin_inff() : -__builtin_inff();
     }
-    const uint32_t nblk = (nq + 31) / 32;
-    __syncthreads();
 
-    // Rows stream through a register double buffer: tile t + 32 is loaded
-    // while tile t is on the matrix cores.
-    RowFrag<VAR> rf;
-    float xn[16];
-    if (W.row_begin < W.row_end) load_rows<VAR>(a, L, W.row_begin, lane, rf);
-    if constexpr (VAR == 1) {
+    // Stage tile row_begin into buffer 0.
+    {
+        const uint4 *src = gA + (uint64_t)(W.row_begin / kMTile) * kU4;
 #pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            const float4 v = *reinterpret_cast<const float4 *>(L.xnorm + W.row_begin + 8 * g + 4 * half);
-            xn[4 * g + 0] = v.x; xn[4 * g + 1] = v.y; xn[4 * g + 2] = v.z; xn[4 * g + 3] = v.w;
+        for (int i = 0; i < kPer; ++i) {
+            const uint32_t e = tid + i * 64 * NW;
+            if (e < (uint32_t)kU4) sA[0][e] = src[e];
         }
     }
-    uint32_t since_sync = 0;
+    __syncthreads();
+
+    uint32_t since_share = 0;
+    uint32_t buf = 0;
 #pragma unroll 1
-    for (uint32_t t = W.row_begin; t < W.row_end; t += kMTile) {
-        RowFrag<VAR> nf;
-        float nxn[16];
+    for (uint32_t t = W.row_begin; t < W.row_end; t += kMTile, buf ^= 1u) {
+        // Issue the next tile's loads now; they land in LDS after this tile's math.
         const uint32_t tn = t + kMTile;
+        uint4 pre[kPer];
         if (tn < W.row_end) {
-            load_rows<VAR>(a, L, tn, lane, nf);
-            if constexpr (VAR == 1) {
+            const uint4 *src = gA + (uint64_t)(tn / kMTile) * kU4;
 #pragma unroll
-                for (int g = 0; g < 4; ++g) {
-                    const float4 v = *reinterpret_cast<const float4 *>(L.xnorm + tn + 8 * g + 4 * half);
-                    nxn[4 * g + 0] = v.x; nxn[4 * g + 1] = v.y; nxn[4 * g + 2] = v.z; nxn[4 * g + 3] = v.w;
-                }
+            for (int i = 0; i < kPer; ++i) {
+                const uint32_t e = tid + i * 64 * NW;
+                if (e < (uint32_t)kU4) pre[i] = src[e];
             }
         }
+        bool hit_any = false;
+        uint64_t hit[kQB];
+#pragma unroll
+        for (int b = 0; b < kQB; ++b) {
+            hit[b] = __ballot(qv[b] && t < qe[b] && t + kMTile > qs[b]);
+            hit_any |= hit[b] != 0;
+        }
         bool appended = false;
+        if (hit_any) {
+            RowFrag<VAR> rf;
+            frag_from_lds<VAR>(sA[buf], lane, rf);
+            float xn[16];
+            if constexpr (VAR == 1) {
+                rf.scale = (int)L.scale[t + col] * 0x01010101;
 #pragma unroll
-        for (int b = 0; b < QB; ++b) {
-            if ((uint32_t)b >= nblk) continue;
-            if (__ballot(qv[b] && t < qe[b] && t + kMTile > qs[b]) == 0) continue;
-            f32x16 acc = tile_mma<VAR>(rf, qf[b]);
-            float sc[16];
+                for (int g = 0; g < 4; ++g) {
+                    const float4 v = *reinterpret_cast<const float4 *>(L.xnorm + t + 8 * g + 4 * half);
+                    xn[4 * g + 0] = v.x; xn[4 * g + 1] = v.y; xn[4 * g + 2] = v.z; xn[4 * g + 3] = v.w;
+                }
+            }
 #pragma unroll
-            for (int j = 0; j < 16; ++j) sc[j] = VAR == 1 ? acc[j] + xn[j] : acc[j];
-            // One compare per lane: the smallest of its 16 scores.
-            float mn = sc[0];
+            for (int b = 0; b < kQB; ++b) {
+                if (hit[b] == 0) continue;
+                f32x16 acc = tile_mma<VAR>(rf, qf[b]);
+                float sc[16];
 #pragma unroll
-            for (int j = 1; j < 16; ++j) mn = fminf(mn, sc[j]);
-            if (__ballot(mn <= tauf[b]) == 0) continue;
-            // Rare path: exact key test and append.
-            appended = true;
-            const bool partial = __ballot(qv[b] && !(t >= qs[b] && t + kMTile <= qe[b])) != 0;
-            const uint32_t qj = 32 * b + col;
-            uint64_t *list = a.lists + (W.slot_begin + qj) * (uint64_t)B;
+                for (int j = 0; j < 16; ++j) sc[j] = VAR == 1 ? acc[j] + xn[j] : acc[j];
+                // One compare per lane: the smallest of its 16 scores.
+                float mn = sc[0];
 #pragma unroll
-            for (int j = 0; j < 16; ++j) {
-                const uint32_t row = t + 8 * (j / 4) + 4 * half + (j % 4);
-                bool ok = qv[b] && sc[j] <= tauf[b];
-                if (partial) ok = ok && row >= qs[b] && row < qe[b];
-                if (ok) {
-                    const uint64_t key = ((uint64_t)f2ord(sc[j]) << 32) | L.ids[row];
-                    if (key < s_tau[qj]) {
-                        const uint32_t pos = atomicAdd(&s_cnt[qj], 1u);
-                        list[pos] = key;
+                for (int j = 1; j < 16; ++j) mn = fminf(mn, sc[j]);
+                if (__ballot(mn <= tauf[b]) == 0) continue;
+                // Rare path: exact key test and append.
+                appended = true;
+                const bool partial = __ballot(qv[b] && !(t >= qs[b] && t + kMTile <= qe[b])) != 0;
+                const uint32_t qj = 32 * b + col;
+                uint64_t *list = a.lists + (W.slot_begin + q0 + qj) * (uint64_t)B;
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    const uint32_t row = t + 8 * (j / 4) + 4 * half + (j % 4);
+                    bool ok = qv[b] && sc[j] <= tauf[b];
+                    if (partial) ok = ok && row >= qs[b] && row < qe[b];
+                    if (ok) {
+                        const uint64_t key = ((uint64_t)f2ord(sc[j]) << 32) | L.ids[row];
+                        if (key < tau_w[qj]) {
+                            const uint32_t pos = atomicAdd(&cnt_w[qj], 1u);
+                            list[pos] = key;
+                        }
                     }
                 }
             }
         }
-        rf = nf;
-        if constexpr (VAR == 1) {
+        // Hand the prefetched tile to LDS.
+        if (tn < W.row_end) {
 #pragma unroll
-            for (int j = 0; j < 16; ++j) xn[j] = nxn[j];
-        }
-        ++since_sync;
-        if (!appended && since_sync < 64) continue;
-        __syncthreads();
-        // Compact every list that could overflow on the next tile, and pull in
-        // thresholds other items of the same queries have published.
-        bool changed = false;
-#pragma unroll 1
-        for (uint32_t j0 = 0; j0 < nq; j0 += 64) {
-            const uint32_t j = j0 + lane;
-            uint64_t need = __ballot(j < nq && s_cnt[j] > (uint32_t)(B - kMTile));
-            while (need) {
-                const uint32_t jj = j0 + (uint32_t)__builtin_ctzll(need);
-                need &= need - 1;
-                uint32_t cnt = s_cnt[jj];
-                uint64_t tau = s_tau[jj];
-                compact_global<B>(a.lists + (W.slot_begin + jj) * (uint64_t)B, cnt, tau, a.K, sbuf);
-                if (lane == 0) {
-                    s_cnt[jj] = cnt;
-                    s_tau[jj] = tau;
-                }
-                changed = true;
-                __syncthreads();
+            for (int i = 0; i < kPer; ++i) {
+                const uint32_t e = tid + i * 64 * NW;
+                if (e < (uint32_t)kU4) sA[buf ^ 1u][e] = pre[i];
             }
         }
-        if (since_sync >= 64) {
-            since_sync = 0;
-            share_thresholds(a.gtau, nq, s_tau, [&](uint32_t j) { return a.iq[W.q_begin + j].qi; });
-            changed = true;
-        }
-        if (changed) {
+        // Wave-local list maintenance (no cross-wave synchronisation).
+        ++since_share;
+        if (appended || since_share >= 64) {
+            WaveSync::sync();
+            bool changed = false;
+            uint64_t need = __ballot(lane < nq && cnt_w[lane] > (uint32_t)(B - kMTile));
+            while (need) {
+                const uint32_t jj = (uint32_t)__builtin_ctzll(need);
+                need &= need - 1;
+                uint32_t cnt = cnt_w[jj];
+                uint64_t tau = tau_w[jj];
+                compact_global<B, WaveSync>(a.lists + (W.slot_begin + q0 + jj) * (uint64_t)B, cnt, tau, a.K, sbuf);
+                if (lane == 0) {
+                    cnt_w[jj] = cnt;
+                    tau_w[jj] = tau;
+                }
+                changed = true;
+                WaveSync::sync();
+            }
+            if (since_share >= 64) {
+                since_share = 0;
+                share_thresholds<WaveSync>(a.gtau, nq, tau_w, qi_of);
+                changed = true;
+            }
+            if (changed) {
 #pragma unroll
-            for (int b = 0; b < QB; ++b)
-                if (qv[b]) tauf[b] = tau_as_float(s_tau[32 * b + col]);
+                for (int b = 0; b < kQB; ++b)
+                    if (qv[b]) tauf[b] = tau_as_float(tau_w[32 * b + col]);
+            }
         }
+        __syncthreads();  // next tile staged; this tile's buffer free for reuse
     }
-    __syncthreads();
+    // Final cut: lists longer than K are sorted and cut; shorter ones stay
+    // unsorted (the merge does not need order).
+    WaveSync::sync();
     for (uint32_t j = 0; j < nq; ++j) {
-        uint32_t cnt = s_cnt[j];
-        uint64_t tau = s_tau[j];
-        compact_global<B>(a.lists + (W.slot_begin + j) * (uint64_t)B, cnt, tau, a.K, sbuf);
-        if (lane == 0) {
-            a.counts[W.slot_begin + j] = cnt;
-            if (tau != kKeyMax) atomicMin((unsigned long long *)&a.gtau[a.iq[W.q_begin + j].qi], tau);
+        uint32_t cnt = cnt_w[j];
+        uint64_t tau = tau_w[j];
+        if (cnt > a.K) {
+            compact_global<B, WaveSync>(a.lists + (W.slot_begin + q0 + j) * (uint64_t)B, cnt, tau, a.K, sbuf);
+            if (lane == 0) tau_w[j] = tau;
         }
+        if (lane == 0) a.counts[W.slot_begin + q0 + j] = cnt;
     }
+    share_thresholds<WaveSync>(a.gtau, nq, tau_w, qi_of);
 }
 
-// At least two waves per SIMD (<= 256 registers) so one wave's loads and
-// selection overlap another's matrix work.
-template <int VAR, int QB, int B>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(QB == 1 ? 3 : 2, 8)))
+template <int VAR, int NW, int B>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2, 8)))
 void scan_mfma_kernel(ScanMfmaArgs a) {
-    __shared__ uint64_t sbuf[B];
-    __shared__ uint64_t s_tau[32 * QB];
-    __shared__ uint32_t s_cnt[32 * QB];
+    __shared__ uint4 sA[2][TileBytes<VAR>::value / 16];
+    __shared__ uint64_t sbuf[NW][B];
+    __shared__ uint64_t s_tau[64 * NW];
+    __shared__ uint32_t s_cnt[64 * NW];
+    __shared__ uint32_t s_idx;
+    uint64_t *my_sbuf = sbuf[threadIdx.x >> 6];
     if (a.queue == nullptr) {
         const WorkItem W = as_const(a.items)[blockIdx.x];
-        scan_mfma_item<VAR, QB, B>(a, W, sbuf, s_tau, s_cnt);
+        scan_mfma_item<VAR, NW, B>(a, W, sA, my_sbuf, s_tau, s_cnt);
         return;
     }
     for (;;) {
-        uint32_t idx = 0;
-        if (threadIdx.x == 0) idx = atomicAdd(a.queue, 1u);
-        idx = __builtin_amdgcn_readfirstlane(__shfl(idx, 0));
+        if (threadIdx.x == 0) s_idx = atomicAdd(a.queue, 1u);
+        __syncthreads();
+        const uint32_t idx = __builtin_amdgcn_readfirstlane(s_idx);
+        __syncthreads();
         if (idx >= a.n_items) break;
         const WorkItem W = as_const(a.items)[idx];
-        scan_mfma_item<VAR, QB, B>(a, W, sbuf, s_tau, s_cnt);
+        scan_mfma_item<VAR, NW, B>(a, W, sA, my_sbuf, s_tau, s_cnt);
         __syncthreads();
     }
 }
 
-template <int VAR, int QB>
-static hipError_t launch_mfma_qb(const ScanMfmaArgs &a, int B, uint64_t grid, hipStream_t st) {
+template <int VAR, int NW>
+static hipError_t launch_mfma_nw(const ScanMfmaArgs &a, int B, uint64_t grid, hipStream_t st) {
     switch (B) {
-    case 256: hipLaunchKernelGGL((scan_mfma_kernel<VAR, QB, 256>), dim3((uint32_t)grid), dim3(64), 0, st, a); break;
-    case 512: hipLaunchKernelGGL((scan_mfma_kernel<VAR, QB, 512>), dim3((uint32_t)grid), dim3(64), 0, st, a); break;
-    case 1024: hipLaunchKernelGGL((scan_mfma_kernel<VAR, QB, 1024>), dim3((uint32_t)grid), dim3(64), 0, st, a); break;
+    case 256: hipLaunchKernelGGL((scan_mfma_kernel<VAR, NW, 256>), dim3((uint32_t)grid), dim3(64 * NW), 0, st, a); break;
+    case 512: hipLaunchKernelGGL((scan_mfma_kernel<VAR, NW, 512>), dim3((uint32_t)grid), dim3(64 * NW), 0, st, a); break;
+    case 1024: hipLaunchKernelGGL((scan_mfma_kernel<VAR, NW, 1024>), dim3((uint32_t)grid), dim3(64 * NW), 0, st, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_scan_mfma(int variant, const ScanMfmaArgs &a, int B, int qb, hipStream_t st) {
+template <int VAR>
+static hipError_t launch_mfma_var(const ScanMfmaArgs &a, int B, int nw, uint64_t grid, hipStream_t st) {
+    if (nw == 1) return launch_mfma_nw<VAR, 1>(a, B, grid, st);
+    if (nw == 2) return launch_mfma_nw<VAR, 2>(a, B, grid, st);
+    return launch_mfma_nw<VAR, 4>(a, B, grid, st);
+}
+
+// nw: waves per workgroup (1, 2 or 4); an item carries at most 64 * nw queries.
+hipError_t launch_scan_mfma(int variant, const ScanMfmaArgs &a, int B, int nw, hipStream_t st) {
     if (a.n_items == 0) return hipSuccess;
+    if (nw != 1 && nw != 2 && nw != 4) return hipErrorInvalidValue;
     uint64_t grid = a.n_items;
     if (a.queue) {
         int dev = 0, cus = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        grid = std::min<uint64_t>((uint64_t)cus * 8, a.n_items);
+        grid = std::min<uint64_t>((uint64_t)cus * (8 / nw), a.n_items);
     }
-    if (qb == 1) {
-        if (variant == 0) return launch_mfma_qb<0, 1>(a, B, grid, st);
-        if (variant == 1) return launch_mfma_qb<1, 1>(a, B, grid, st);
-        return launch_mfma_qb<2, 1>(a, B, grid, st);
-    }
-    if (variant == 0) return launch_mfma_qb<0, 2>(a, B, grid, st);
-    if (variant == 1) return launch_mfma_qb<1, 2>(a, B, grid, st);
-    return launch_mfma_qb<2, 2>(a, B, grid, st);
+    if (variant == 0) return launch_mfma_var<0>(a, B, nw, grid, st);
+    if (variant == 1) return launch_mfma_var<1>(a, B, nw, grid, st);
+    return launch_mfma_var<2>(a, B, nw, grid, st);
 }
 
 }  // namespace fbf

@@ -31,11 +31,26 @@ __device__ inline uint64_t uniform_u64(uint64_t v) {
     return ((uint64_t)hi << 32) | lo;
 }
 
-// Ascending bitonic sort of B keys held in LDS by one wave.  The workgroup is
-// this single wave, so __syncthreads() only orders the LDS traffic.
-template <int B>
+// Synchronisation policies for the list routines.  BlockSync: the workgroup
+// is one wave, __syncthreads() orders its LDS and global traffic.  WaveSync:
+// a wave of a larger workgroup working on data only it touches: wait for all
+// its outstanding memory operations, then a wave barrier (no cross-wave wait).
+struct BlockSync {
+    __device__ static void sync() { __syncthreads(); }
+};
+struct WaveSync {
+    __device__ static void sync() {
+        __asm__ volatile("" ::: "memory");
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+        __asm__ volatile("" ::: "memory");
+    }
+};
+
+// Ascending bitonic sort of B keys held in LDS by one wave.
+template <int B, class Sync = BlockSync>
 __device__ inline void wave_sort_lds(uint64_t *s) {
-    const uint32_t lane = threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63;
 #pragma unroll 1
     for (int size = 2; size <= B; size <<= 1) {
 #pragma unroll 1
@@ -52,25 +67,25 @@ __device__ inline void wave_sort_lds(uint64_t *s) {
                     s[j] = x;
                 }
             }
-            __syncthreads();
+            Sync::sync();
         }
     }
 }
 
 // A query's candidate list in global memory (list) with its count and
 // threshold.  The LDS buffer sbuf (B keys) is scratch for compaction.
-template <int B>
+template <int B, class Sync = BlockSync>
 __device__ inline void compact_global(uint64_t *list, uint32_t &cnt, uint64_t &tau, uint32_t K,
                                       uint64_t *sbuf) {
-    const uint32_t lane = threadIdx.x;
-    __syncthreads();  // list stores of this wave complete before they are read back
+    const uint32_t lane = threadIdx.x & 63;
+    Sync::sync();  // list stores of this wave complete before they are read back
 #pragma unroll
     for (int i0 = 0; i0 < B; i0 += 64) {
         const uint32_t i = i0 + lane;
         sbuf[i] = i < cnt ? list[i] : kKeyMax;
     }
-    __syncthreads();
-    wave_sort_lds<B>(sbuf);
+    Sync::sync();
+    wave_sort_lds<B, Sync>(sbuf);
     const uint32_t keep = cnt < K ? cnt : K;
 #pragma unroll
     for (int i0 = 0; i0 < B; i0 += 64) {
@@ -79,7 +94,7 @@ __device__ inline void compact_global(uint64_t *list, uint32_t &cnt, uint64_t &t
     }
     tau = (keep == K) ? uniform_u64(sbuf[K - 1]) : kKeyMax;
     cnt = keep;
-    __syncthreads();
+    Sync::sync();
 }
 
 // The same with the list itself in LDS (merge kernels).
@@ -127,16 +142,16 @@ __device__ inline void offer_lds(uint64_t *list, uint32_t &cnt, uint64_t &tau, u
 // fetches).  Valid because a key at or above the K-th smallest key of ANY
 // subset of a query's candidates cannot be in that query's top-K.  s_tau[j]
 // is the item's threshold of its query j, iq_qi(j) the query's batch index.
-template <class QiFn>
+template <class Sync = BlockSync, class QiFn>
 __device__ inline void share_thresholds(uint64_t *gtau, uint32_t nq, uint64_t *s_tau, QiFn iq_qi) {
     if (gtau == nullptr) return;
-    __syncthreads();
-    for (uint32_t j = threadIdx.x; j < nq; j += 64) {
+    Sync::sync();
+    for (uint32_t j = threadIdx.x & 63; j < nq; j += 64) {
         const uint64_t mine = s_tau[j];
         const uint64_t old = atomicMin((unsigned long long *)&gtau[iq_qi(j)], (unsigned long long)mine);
         s_tau[j] = old < mine ? old : mine;
     }
-    __syncthreads();
+    Sync::sync();
 }
 
 }  // namespace fbf
