@@ -44,6 +44,7 @@ def parse():
     p.add_argument("--chunk-rows", type=int, default=0)
     p.add_argument("--group-queries", type=int, default=0)
     p.add_argument("--persistent", action="store_true")
+    p.add_argument("--prepass-div", type=int, default=0)
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--recall-sample", type=int, default=16, help="queries checked against FP64")
@@ -82,7 +83,7 @@ def main() -> int:
     out = (torch.empty((nq, a.k), dtype=torch.int32, device=dev),
            torch.empty((nq, a.k), dtype=torch.float32, device=dev))
     kw = dict(k=a.k, mode=mode, kprime=a.kprime, chunk_rows=a.chunk_rows, group_queries=a.group_queries,
-              persistent=a.persistent, out=out)
+              persistent=a.persistent, out=out, prepass_div=a.prepass_div)
 
     for i in range(a.warmup):
         ds.search(qd, metas, **kw)

@@ -61,17 +61,19 @@ typedef enum fbf_mode {
 
 typedef struct fbf_dataset fbf_dataset;
 
-/* Search tuning.  Zero fields take the library defaults (or the autotuned
- * values recorded by fbf_autotune).  kprime is the over-fetch depth of the
- * approximate modes (ignored by FBF_MODE_EXACT_F32); chunk_rows is the rows
- * per work item; group_queries the queries sharing one streamed tile. */
+/* Search tuning.  Zero fields take the library defaults.  kprime is the
+ * over-fetch depth of the approximate modes (ignored by FBF_MODE_EXACT_F32);
+ * chunk_rows is the rows per work item; group_queries the queries sharing one
+ * streamed tile (<= 16 for the VALU scans, <= 256 for the MFMA scans). */
 typedef struct fbf_search_opts {
     uint32_t mode;        /* fbf_mode */
     uint32_t kprime;      /* 0 -> 4k */
     uint32_t chunk_rows;  /* 0 -> default */
     uint32_t group_queries;
     uint32_t persistent;  /* 1 -> one persistent work-queue launch for the scan */
-    uint32_t reserved[3];
+    uint32_t prepass_div; /* approximate modes: scan the first 1/prepass_div of each large range
+                             first to seed the per-query thresholds (0 -> default 16, 1 -> off) */
+    uint32_t reserved[2];
 } fbf_search_opts;
 
 /* Create a device-resident dataset on the current HIP device from host

@@ -485,6 +485,30 @@ static void make_plan(const std::vector<QRange> &qr, uint32_t chunk_rows, uint32
     for (auto &so : slot_owner) P.q_slots[fill[so.first]++] = so.second;
 }
 
+// Append plan B (same batch) after plan A: B's items follow A's, its slots
+// are renumbered after A's, and each query's slot list holds A's then B's.
+static void concat_plans(Plan &A, const Plan &B, uint64_t nq) {
+    const uint64_t slotsA = A.q_slots.size();
+    const uint32_t iqA = (uint32_t)A.iq.size();
+    for (WorkItem W : B.items) {
+        W.slot_begin += slotsA;
+        W.q_begin += iqA;
+        A.items.push_back(W);
+    }
+    A.iq.insert(A.iq.end(), B.iq.begin(), B.iq.end());
+    std::vector<uint64_t> off(nq + 1, 0), qs;
+    qs.reserve(slotsA + B.q_slots.size());
+    for (uint64_t i = 0; i < nq; ++i) {
+        for (uint64_t k = A.slot_off[i]; k < A.slot_off[i + 1]; ++k) qs.push_back(A.q_slots[k]);
+        for (uint64_t k = B.slot_off[i]; k < B.slot_off[i + 1]; ++k) qs.push_back(B.q_slots[k] + slotsA);
+        off[i + 1] = qs.size();
+    }
+    A.slot_off.swap(off);
+    A.q_slots.swap(qs);
+    A.pairs += B.pairs;
+    A.pairs_computed += B.pairs_computed;
+}
+
 // Reserve scratch for a plan with list capacity B and copy the plan to the
 // device through pinned memory (asynchronous on st).
 static fbf_status upload_plan(Scratch &S, const Plan &P, uint64_t nq, int B, hipStream_t st) {
@@ -512,14 +536,6 @@ static fbf_status upload_plan(Scratch &S, const Plan &P, uint64_t nq, int B, hip
     HIP_TRY(hipMemcpyAsync(S.d_slot_off.p, h + b_items + b_iq, b_off, hipMemcpyHostToDevice, st));
     if (b_qs) HIP_TRY(hipMemcpyAsync(S.d_qslots.p, h + b_items + b_iq + b_off, b_qs, hipMemcpyHostToDevice, st));
     return FBF_OK;
-}
-
-// Range of items of one layout (items are sorted by layout).
-static void layout_run(const Plan &P, uint32_t L, uint64_t &i0, uint64_t &i1) {
-    i0 = 0;
-    while (i0 < P.items.size() && P.items[i0].layout != L) ++i0;
-    i1 = i0;
-    while (i1 < P.items.size() && P.items[i1].layout == L) ++i1;
 }
 
 static MergeArgs merge_args(const Scratch &S, int B, uint64_t nq, uint32_t k, uint32_t kprime) {
@@ -591,9 +607,37 @@ extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, cons
     std::vector<QRange> qr;
     s = compute_ranges(ds, metas, nq, qr);
     if (s != FBF_OK) return s;
+    // Approximate modes: the first 1/prepass_div of every large range is
+    // scanned first (its own launch), so the per-query thresholds it publishes
+    // gate the scan of the rest from the first tile on.
+    const uint32_t pdiv = exact ? 1u : (o.prepass_div ? o.prepass_div : 16u);
     Plan P;
-    make_plan(qr, chunk, gmax, P);
-    order_items(P, !mfma);
+    uint64_t n_items_a = 0;
+    if (pdiv > 1) {
+        std::vector<QRange> qa = qr, qb = qr;
+        const uint64_t min_prefix = std::max<uint64_t>(64 * 1024, 8ull * K);
+        for (uint64_t i = 0; i < nq; ++i) {
+            const uint64_t len = qr[i].e - qr[i].s;
+            const uint64_t pre = std::max<uint64_t>(min_prefix, (len + pdiv - 1) / pdiv);
+            if (len > 2 * pre) {
+                qa[i].e = qr[i].s + (uint32_t)pre;
+                qb[i].s = qa[i].e;
+            } else {
+                qb[i].s = qb[i].e;  // whole range in the first pass
+            }
+        }
+        make_plan(qa, chunk, gmax, P);
+        order_items(P, !mfma);
+        n_items_a = P.items.size();
+        Plan PB;
+        make_plan(qb, chunk, gmax, PB);
+        order_items(PB, !mfma);
+        concat_plans(P, PB, nq);
+    } else {
+        make_plan(qr, chunk, gmax, P);
+        order_items(P, !mfma);
+        n_items_a = P.items.size();
+    }
     ds->last_pairs = P.pairs;
     ds->last_pairs_computed = P.pairs_computed;
     ds->last_items = P.items.size();
@@ -630,13 +674,20 @@ extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, cons
         a.counts = S.d_counts.as<uint32_t>();
         a.gtau = gtau;
         a.K = K;
-        a.queue = queue;
-        if (queue) HIP_TRY(launch_fill_u32(queue, 0u, 1, st));
         const int nw = gmax <= 64 ? 1 : gmax <= 128 ? 2 : 4;
-        ProfEvent ev;
-        record_begin(ds, 0, st, ev);
-        HIP_TRY(launch_scan_mfma(variant, a, B, nw, st));
-        record_end(ds, st, ev);
+        // Prefix pass, then the rest.
+        const uint64_t bounds[3] = {0, n_items_a, P.items.size()};
+        for (int ph = 0; ph < 2; ++ph) {
+            if (bounds[ph + 1] == bounds[ph]) continue;
+            a.items = S.d_items.as<WorkItem>() + bounds[ph];
+            a.n_items = bounds[ph + 1] - bounds[ph];
+            a.queue = queue;
+            if (queue) HIP_TRY(launch_fill_u32(queue, 0u, 1, st));
+            ProfEvent ev;
+            record_begin(ds, 0, st, ev);
+            HIP_TRY(launch_scan_mfma(variant, a, B, nw, st));
+            record_end(ds, st, ev);
+        }
     } else {
         int8_t *qcodes = nullptr;
         if (o.mode == FBF_MODE_SQ8SYM) {
@@ -644,10 +695,17 @@ extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, cons
             qcodes = S.d_q.as<int8_t>();
             HIP_TRY(launch_sym_encode(queries_dev, nullptr, nq, nq, ds->alpha, qcodes, st));
         }
-        for (uint32_t L = 0; L < 2; ++L) {
-            uint64_t i0, i1;
-            layout_run(P, L, i0, i1);
-            if (i1 == i0) continue;
+        // Runs of equal layout in item order (per pass: each pass is sorted by layout).
+        std::vector<std::pair<uint64_t, uint64_t>> runs;
+        for (uint64_t i = 0; i < P.items.size();) {
+            uint64_t j = i + 1;
+            while (j < P.items.size() && P.items[j].layout == P.items[i].layout && j != n_items_a) ++j;
+            runs.emplace_back(i, j);
+            i = j;
+        }
+        for (auto &run : runs) {
+            const uint64_t i0 = run.first, i1 = run.second;
+            const uint32_t L = P.items[i0].layout;
             if (queue) HIP_TRY(launch_fill_u32(queue, 0u, 1, st));
             ProfEvent ev;
             record_begin(ds, 0, st, ev);

@@ -79,12 +79,16 @@ __device__ void scan_f32_item(const ScanArgs &a, const WorkItem &W, uint64_t *sb
                               uint64_t *s_tau, uint32_t *s_cnt, float4 *sq) {
     const uint32_t lane = threadIdx.x;
     const uint32_t nq = W.q_count;
-    for (uint32_t j = 0; j < nq; ++j) {
-        s_tau[j] = kKeyMax;  // every lane writes the same value and reads only its own writes
-        s_cnt[j] = 0;
-    }
     const float4 *soa = reinterpret_cast<const float4 *>(a.soa);
     cptr<ItemQuery> iq = as_const(a.iq) + W.q_begin;
+    // Start from the thresholds earlier items of the same queries published.
+    for (uint32_t j = 0; j < nq; ++j) {
+        // every lane writes the same value and reads only its own writes
+        s_tau[j] = a.gtau ? uniform_u64(__hip_atomic_load(&a.gtau[iq[j].qi], __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_AGENT))
+                          : kKeyMax;
+        s_cnt[j] = 0;
+    }
     // The item's query vectors, staged once in LDS; the scan reads them at
     // wave-uniform addresses (LDS broadcast).
     __syncthreads();

@@ -440,10 +440,13 @@ __device__ void scan_mfma_item(const ScanMfmaArgs &a, const WorkItem &W, uint4 (
     const uint4 *gA = reinterpret_cast<const uint4 *>(L.tiles);
     auto qi_of = [&](uint32_t j) { return a.iq[W.q_begin + q0 + j].qi; };
 
+    // Start from the thresholds earlier items (or the prefix pass) published.
     for (uint32_t j = lane; j < nq; j += 64) {
-        tau_w[j] = kKeyMax;
+        tau_w[j] = a.gtau ? __hip_atomic_load(&a.gtau[qi_of(j)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                          : kKeyMax;
         cnt_w[j] = 0;
     }
+    WaveSync::sync();
     QFrag<VAR> qf[kQB];
     float tauf[kQB];
     uint32_t qs[kQB], qe[kQB];
@@ -461,19 +464,23 @@ __device__ void scan_mfma_item(const ScanMfmaArgs &a, const WorkItem &W, uint4 (
             qe[b] = e.e;
         }
         load_query<VAR>(a, qv[b], qi, lane, qf[b]);
-        tauf[b] = qv[b] ? __builtin_inff() : -__builtin_inff();
+        tauf[b] = qv[b] ? tau_as_float(tau_w[j]) : -__builtin_inff();
     }
 
-    // Stage tile row_begin into buffer 0.
-    {
+    // One wave: rows go straight from HBM/L2 into a register double buffer.
+    // Several waves: the tile is staged once in LDS for all of them.
+    RowFrag<VAR> rf_cur, rf_next;
+    if constexpr (NW == 1) {
+        if (W.row_begin < W.row_end) frag_from_lds<VAR>(gA + (uint64_t)(W.row_begin / kMTile) * kU4, lane, rf_cur);
+    } else {
         const uint4 *src = gA + (uint64_t)(W.row_begin / kMTile) * kU4;
 #pragma unroll
         for (int i = 0; i < kPer; ++i) {
             const uint32_t e = tid + i * 64 * NW;
             if (e < (uint32_t)kU4) sA[0][e] = src[e];
         }
+        __syncthreads();
     }
-    __syncthreads();
 
     uint32_t since_share = 0;
     uint32_t buf = 0;
@@ -484,10 +491,14 @@ __device__ void scan_mfma_item(const ScanMfmaArgs &a, const WorkItem &W, uint4 (
         uint4 pre[kPer];
         if (tn < W.row_end) {
             const uint4 *src = gA + (uint64_t)(tn / kMTile) * kU4;
+            if constexpr (NW == 1) {
+                frag_from_lds<VAR>(src, lane, rf_next);
+            } else {
 #pragma unroll
-            for (int i = 0; i < kPer; ++i) {
-                const uint32_t e = tid + i * 64 * NW;
-                if (e < (uint32_t)kU4) pre[i] = src[e];
+                for (int i = 0; i < kPer; ++i) {
+                    const uint32_t e = tid + i * 64 * NW;
+                    if (e < (uint32_t)kU4) pre[i] = src[e];
+                }
             }
         }
         bool hit_any = false;
@@ -500,7 +511,8 @@ __device__ void scan_mfma_item(const ScanMfmaArgs &a, const WorkItem &W, uint4 (
         bool appended = false;
         if (hit_any) {
             RowFrag<VAR> rf;
-            frag_from_lds<VAR>(sA[buf], lane, rf);
+            if constexpr (NW == 1) rf = rf_cur;
+            else frag_from_lds<VAR>(sA[buf], lane, rf);
             float xn[16];
             if constexpr (VAR == 1) {
                 rf.scale = (int)L.scale[t + col] * 0x01010101;
@@ -542,8 +554,10 @@ __device__ void scan_mfma_item(const ScanMfmaArgs &a, const WorkItem &W, uint4 (
                 }
             }
         }
-        // Hand the prefetched tile to LDS.
-        if (tn < W.row_end) {
+        // Hand the prefetched tile over.
+        if constexpr (NW == 1) {
+            rf_cur = rf_next;
+        } else if (tn < W.row_end) {
 #pragma unroll
             for (int i = 0; i < kPer; ++i) {
                 const uint32_t e = tid + i * 64 * NW;
@@ -580,7 +594,7 @@ __device__ void scan_mfma_item(const ScanMfmaArgs &a, const WorkItem &W, uint4 (
                     if (qv[b]) tauf[b] = tau_as_float(tau_w[32 * b + col]);
             }
         }
-        __syncthreads();  // next tile staged; this tile's buffer free for reuse
+        if constexpr (NW > 1) __syncthreads();  // next tile staged; this tile's buffer free
     }
     // Final cut: lists longer than K are sorted and cut; shorter ones stay
     // unsorted (the merge does not need order).

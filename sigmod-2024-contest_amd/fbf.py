@@ -48,8 +48,8 @@ class FbfError(RuntimeError):
 
 class SearchOpts(C.Structure):
     _fields_ = [("mode", C.c_uint32), ("kprime", C.c_uint32), ("chunk_rows", C.c_uint32),
-                ("group_queries", C.c_uint32), ("persistent", C.c_uint32),
-                ("reserved", C.c_uint32 * 3)]
+                ("group_queries", C.c_uint32), ("persistent", C.c_uint32), ("prepass_div", C.c_uint32),
+                ("reserved", C.c_uint32 * 2)]
 
 
 _lib = None
@@ -152,18 +152,19 @@ class Dataset:
         return c.value, s.value
 
     @staticmethod
-    def _opts(mode, kprime, chunk_rows, group_queries, persistent) -> SearchOpts:
+    def _opts(mode, kprime, chunk_rows, group_queries, persistent, prepass_div=0) -> SearchOpts:
         o = SearchOpts()
         o.mode = MODES[mode] if isinstance(mode, str) else int(mode)
         o.kprime = kprime
         o.chunk_rows = chunk_rows
         o.group_queries = group_queries
         o.persistent = int(persistent)
+        o.prepass_div = prepass_div
         return o
 
     def search(self, queries, metas: np.ndarray, k: int = 100, mode="exact", kprime: int = 0,
                chunk_rows: int = 0, group_queries: int = 0, persistent: bool = False,
-               out=None, stream=None):
+               out=None, stream=None, prepass_div: int = 0):
         """queries: torch CUDA float32 [nq, d]; metas: numpy float32 [nq, 4].
         Returns torch (ids int32 view of uint32 [nq, k], dists float32 [nq, k])
         on the queries' device; asynchronous on ``stream`` (torch stream or None
@@ -178,21 +179,21 @@ class Dataset:
         else:
             ids, dists = out
         st = (stream or torch.cuda.current_stream(queries.device)).cuda_stream
-        o = self._opts(mode, kprime, chunk_rows, group_queries, persistent)
+        o = self._opts(mode, kprime, chunk_rows, group_queries, persistent, prepass_div)
         _check(lib().fbf_search(self.h, _P(_dev_ptr(queries)), _np_ptr(metas), nq, k, C.byref(o),
                                 _P(_dev_ptr(ids)), _P(_dev_ptr(dists)), _P(st)), "fbf_search")
         return ids, dists
 
     def search_host(self, queries: np.ndarray, metas: np.ndarray, k: int = 100, mode="exact",
                     kprime: int = 0, chunk_rows: int = 0, group_queries: int = 0,
-                    persistent: bool = False):
+                    persistent: bool = False, prepass_div: int = 0):
         """Host-memory form: returns numpy (ids uint32 [nq, k], dists float32 [nq, k])."""
         queries = np.ascontiguousarray(queries, np.float32)
         metas = np.ascontiguousarray(metas, np.float32)
         nq = queries.shape[0]
         ids = np.zeros((nq, k), np.uint32)
         dists = np.zeros((nq, k), np.float32)
-        o = self._opts(mode, kprime, chunk_rows, group_queries, persistent)
+        o = self._opts(mode, kprime, chunk_rows, group_queries, persistent, prepass_div)
         _check(lib().fbf_search_host(self.h, _np_ptr(queries), _np_ptr(metas), nq, k, C.byref(o),
                                      _np_ptr(ids), _np_ptr(dists)), "fbf_search_host")
         return ids, dists
