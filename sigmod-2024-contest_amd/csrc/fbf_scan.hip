@@ -75,7 +75,7 @@ __device__ inline void load_row(const float4 *__restrict__ soa, uint32_t tile_ro
 }
 
 template <int B>
-__device__ void scan_f32_item(const ScanArgs &a, const WorkItem &W, uint64_t *sbuf,
+__device__ __attribute__((always_inline)) void scan_f32_item(const ScanArgs &a, const WorkItem &W, uint64_t *sbuf,
                               uint64_t *s_tau, uint32_t *s_cnt, float4 *sq) {
     const uint32_t lane = threadIdx.x;
     const uint32_t nq = W.q_count;

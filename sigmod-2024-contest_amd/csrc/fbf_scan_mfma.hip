@@ -26,6 +26,8 @@
 //     row 8*(j/4) + 4*(l/32) + j%4, column l%32.
 //   v_mfma_scale_f32_32x32x64_f8f6f4 (e4m3): lane l holds row / column l%32,
 //     k = 32*(l/32) + e, e = 0..31; same D layout.
+#include <utility>
+
 #include "fbf_select.hpp"
 
 namespace fbf {
@@ -425,7 +427,7 @@ __device__ inline float tau_as_float(uint64_t tau) {
 constexpr int kQB = 2;  // 32-query blocks per wave
 
 template <int VAR, int NW, int B>
-__device__ void scan_mfma_item(const ScanMfmaArgs &a, const WorkItem &W, uint4 (*sA)[TileBytes<VAR>::value / 16],
+__device__ __attribute__((always_inline)) void scan_mfma_item(const ScanMfmaArgs &a, const WorkItem &W, uint4 (*sA)[TileBytes<VAR>::value / 16],
                                uint64_t *sbuf, uint64_t *s_tau, uint32_t *s_cnt) {
     constexpr int kU4 = TileBytes<VAR>::value / 16;
     constexpr int kPer = (kU4 + 64 * NW - 1) / (64 * NW);  // staging loads per thread
@@ -467,135 +469,164 @@ __device__ void scan_mfma_item(const ScanMfmaArgs &a, const WorkItem &W, uint4 (
         tauf[b] = qv[b] ? tau_as_float(tau_w[j]) : -__builtin_inff();
     }
 
-    // One wave: rows go straight from HBM/L2 into a register double buffer.
-    // Several waves: the tile is staged once in LDS for all of them.
-    RowFrag<VAR> rf_cur, rf_next;
-    if constexpr (NW == 1) {
-        if (W.row_begin < W.row_end) frag_from_lds<VAR>(gA + (uint64_t)(W.row_begin / kMTile) * kU4, lane, rf_cur);
-    } else {
-        const uint4 *src = gA + (uint64_t)(W.row_begin / kMTile) * kU4;
+    // Rows stream through LDS.  kDepth tiles are in flight in registers (each
+    // thread holds kPer 16-byte pieces of every tile), one tile sits in LDS for
+    // the matrix cores, and a tile's loads are issued kDepth tiles before its
+    // math so their latency hides behind the work in between.  The ring is
+    // indexed with compile-time slots (the loop body is unrolled kDepth times).
+    constexpr int kDepth = kPer <= 1 ? 6 : kPer <= 2 ? 4 : kPer <= 4 ? 2 : 1;
+    const uint32_t ntile = (W.row_end - W.row_begin + kMTile - 1) / kMTile;
+    // Plain vector type for the ring: the HIP uint4 wrapper struct keeps the
+    // compiler from promoting the array to registers.
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 *gbase = reinterpret_cast<const u32x4 *>(gA) + (uint64_t)(W.row_begin / kMTile) * kU4;
+    u32x4 (*sAv)[kU4] = reinterpret_cast<u32x4 (*)[kU4]>(sA);
+    u32x4 ring[kDepth][kPer];
+    // Tile 0 straight to LDS, tiles 1..kDepth into the ring.
+    if (ntile > 0) {
 #pragma unroll
         for (int i = 0; i < kPer; ++i) {
             const uint32_t e = tid + i * 64 * NW;
-            if (e < (uint32_t)kU4) sA[0][e] = src[e];
+            if (e < (uint32_t)kU4) sAv[0][e] = gbase[e];
         }
-        __syncthreads();
     }
-
-    uint32_t since_share = 0;
-    uint32_t buf = 0;
-#pragma unroll 1
-    for (uint32_t t = W.row_begin; t < W.row_end; t += kMTile, buf ^= 1u) {
-        // Issue the next tile's loads now; they land in LDS after this tile's math.
-        const uint32_t tn = t + kMTile;
-        uint4 pre[kPer];
-        if (tn < W.row_end) {
-            const uint4 *src = gA + (uint64_t)(tn / kMTile) * kU4;
-            if constexpr (NW == 1) {
-                frag_from_lds<VAR>(src, lane, rf_next);
-            } else {
 #pragma unroll
-                for (int i = 0; i < kPer; ++i) {
-                    const uint32_t e = tid + i * 64 * NW;
-                    if (e < (uint32_t)kU4) pre[i] = src[e];
-                }
-            }
-        }
-        bool hit_any = false;
-        uint64_t hit[kQB];
-#pragma unroll
-        for (int b = 0; b < kQB; ++b) {
-            hit[b] = __ballot(qv[b] && t < qe[b] && t + kMTile > qs[b]);
-            hit_any |= hit[b] != 0;
-        }
-        bool appended = false;
-        if (hit_any) {
-            RowFrag<VAR> rf;
-            if constexpr (NW == 1) rf = rf_cur;
-            else frag_from_lds<VAR>(sA[buf], lane, rf);
-            float xn[16];
-            if constexpr (VAR == 1) {
-                rf.scale = (int)L.scale[t + col] * 0x01010101;
-#pragma unroll
-                for (int g = 0; g < 4; ++g) {
-                    const float4 v = *reinterpret_cast<const float4 *>(L.xnorm + t + 8 * g + 4 * half);
-                    xn[4 * g + 0] = v.x; xn[4 * g + 1] = v.y; xn[4 * g + 2] = v.z; xn[4 * g + 3] = v.w;
-                }
-            }
-#pragma unroll
-            for (int b = 0; b < kQB; ++b) {
-                if (hit[b] == 0) continue;
-                f32x16 acc = tile_mma<VAR>(rf, qf[b]);
-                float sc[16];
-#pragma unroll
-                for (int j = 0; j < 16; ++j) sc[j] = VAR == 1 ? acc[j] + xn[j] : acc[j];
-                // One compare per lane: the smallest of its 16 scores.
-                float mn = sc[0];
-#pragma unroll
-                for (int j = 1; j < 16; ++j) mn = fminf(mn, sc[j]);
-                if (__ballot(mn <= tauf[b]) == 0) continue;
-                // Rare path: exact key test and append.
-                appended = true;
-                const bool partial = __ballot(qv[b] && !(t >= qs[b] && t + kMTile <= qe[b])) != 0;
-                const uint32_t qj = 32 * b + col;
-                uint64_t *list = a.lists + (W.slot_begin + q0 + qj) * (uint64_t)B;
-#pragma unroll
-                for (int j = 0; j < 16; ++j) {
-                    const uint32_t row = t + 8 * (j / 4) + 4 * half + (j % 4);
-                    bool ok = qv[b] && sc[j] <= tauf[b];
-                    if (partial) ok = ok && row >= qs[b] && row < qe[b];
-                    if (ok) {
-                        const uint64_t key = ((uint64_t)f2ord(sc[j]) << 32) | L.ids[row];
-                        if (key < tau_w[qj]) {
-                            const uint32_t pos = atomicAdd(&cnt_w[qj], 1u);
-                            list[pos] = key;
-                        }
-                    }
-                }
-            }
-        }
-        // Hand the prefetched tile over.
-        if constexpr (NW == 1) {
-            rf_cur = rf_next;
-        } else if (tn < W.row_end) {
+    for (int d = 0; d < kDepth; ++d) {
+        const uint32_t m = 1 + d;
+        if (m < ntile) {
 #pragma unroll
             for (int i = 0; i < kPer; ++i) {
                 const uint32_t e = tid + i * 64 * NW;
-                if (e < (uint32_t)kU4) sA[buf ^ 1u][e] = pre[i];
+                if (e < (uint32_t)kU4) ring[(1 + d) % kDepth][i] = gbase[(uint64_t)m * kU4 + e];
             }
         }
-        // Wave-local list maintenance (no cross-wave synchronisation).
-        ++since_share;
-        if (appended || since_share >= 64) {
-            WaveSync::sync();
-            bool changed = false;
-            uint64_t need = __ballot(lane < nq && cnt_w[lane] > (uint32_t)(B - kMTile));
-            while (need) {
-                const uint32_t jj = (uint32_t)__builtin_ctzll(need);
-                need &= need - 1;
-                uint32_t cnt = cnt_w[jj];
-                uint64_t tau = tau_w[jj];
-                compact_global<B, WaveSync>(a.lists + (W.slot_begin + q0 + jj) * (uint64_t)B, cnt, tau, a.K, sbuf);
-                if (lane == 0) {
-                    cnt_w[jj] = cnt;
-                    tau_w[jj] = tau;
-                }
-                changed = true;
-                WaveSync::sync();
-            }
-            if (since_share >= 64) {
-                since_share = 0;
-                share_thresholds<WaveSync>(a.gtau, nq, tau_w, qi_of);
-                changed = true;
-            }
-            if (changed) {
-#pragma unroll
-                for (int b = 0; b < kQB; ++b)
-                    if (qv[b]) tauf[b] = tau_as_float(tau_w[32 * b + col]);
-            }
-        }
-        if constexpr (NW > 1) __syncthreads();  // next tile staged; this tile's buffer free
     }
+    __syncthreads();
+
+    uint32_t since_share = 0;
+    // The loop body, once per ring slot with a literal slot number so the
+    // ring stays in registers (a runtime index would put it in scratch).
+#define FBF_TILE_STEP(U)                                                                              \
+    {                                                                                                 \
+        const uint32_t m = m0 + (U);                                                                          \
+        if (m >= ntile) goto tiles_done;                                                                      \
+        const uint32_t t = W.row_begin + m * kMTile;                                                          \
+        bool hit_any = false;                                                                                 \
+        uint64_t hit[kQB];                                                                                    \
+_Pragma("unroll")                                                                                             \
+        for (int b = 0; b < kQB; ++b) {                                                                       \
+            hit[b] = __ballot(qv[b] && t < qe[b] && t + kMTile > qs[b]);                                      \
+            hit_any |= hit[b] != 0;                                                                           \
+        }                                                                                                     \
+        bool appended = false;                                                                                \
+        if (hit_any) {                                                                                        \
+            RowFrag<VAR> rf;                                                                                  \
+            frag_from_lds<VAR>(sA[m & 1u], lane, rf);                                                         \
+            float xn[16];                                                                                     \
+            if constexpr (VAR == 1) {                                                                         \
+                rf.scale = (int)L.scale[t + col] * 0x01010101;                                                \
+_Pragma("unroll")                                                                                             \
+                for (int g = 0; g < 4; ++g) {                                                                 \
+                    const float4 v = *reinterpret_cast<const float4 *>(L.xnorm + t + 8 * g + 4 * half);       \
+                    xn[4 * g + 0] = v.x; xn[4 * g + 1] = v.y; xn[4 * g + 2] = v.z; xn[4 * g + 3] = v.w;       \
+                }                                                                                             \
+            }                                                                                                 \
+_Pragma("unroll")                                                                                             \
+            for (int b = 0; b < kQB; ++b) {                                                                   \
+                if (hit[b] == 0) continue;                                                                    \
+                f32x16 acc = tile_mma<VAR>(rf, qf[b]);                                                        \
+                float sc[16];                                                                                 \
+_Pragma("unroll")                                                                                             \
+                for (int j = 0; j < 16; ++j) sc[j] = VAR == 1 ? acc[j] + xn[j] : acc[j];                      \
+/* One compare per lane: the smallest of its 16 scores. */                                                    \
+                float mn = sc[0];                                                                             \
+_Pragma("unroll")                                                                                             \
+                for (int j = 1; j < 16; ++j) mn = fminf(mn, sc[j]);                                           \
+                if (__ballot(mn <= tauf[b]) == 0) continue;                                                   \
+/* Rare path: exact key test and append. */                                                                   \
+                appended = true;                                                                              \
+                const bool partial = __ballot(qv[b] && !(t >= qs[b] && t + kMTile <= qe[b])) != 0;            \
+                const uint32_t qj = 32 * b + col;                                                             \
+                uint64_t *list = a.lists + (W.slot_begin + q0 + qj) * (uint64_t)B;                            \
+_Pragma("unroll")                                                                                             \
+                for (int j = 0; j < 16; ++j) {                                                                \
+                    const uint32_t row = t + 8 * (j / 4) + 4 * half + (j % 4);                                \
+                    bool ok = qv[b] && sc[j] <= tauf[b];                                                      \
+                    if (partial) ok = ok && row >= qs[b] && row < qe[b];                                      \
+                    if (ok) {                                                                                 \
+                        const uint64_t key = ((uint64_t)f2ord(sc[j]) << 32) | L.ids[row];                     \
+                        if (key < tau_w[qj]) {                                                                \
+                            const uint32_t pos = atomicAdd(&cnt_w[qj], 1u);                                   \
+                            list[pos] = key;                                                                  \
+                        }                                                                                     \
+                    }                                                                                         \
+                }                                                                                             \
+            }                                                                                                 \
+        }                                                                                                     \
+/* Tile m + 1 (issued kDepth tiles ago) goes to LDS; its ring slot */                                         \
+/* takes the loads of tile m + 1 + kDepth. */                                                                 \
+        {                                                                                                     \
+            const int slot = ((U) + 1) % kDepth;                                                              \
+            if (m + 1 < ntile) {                                                                              \
+_Pragma("unroll")                                                                                             \
+                for (int i = 0; i < kPer; ++i) {                                                              \
+                    const uint32_t e = tid + i * 64 * NW;                                                     \
+                    if (e < (uint32_t)kU4) sAv[(m + 1) & 1u][e] = ring[slot][i];                               \
+                }                                                                                             \
+            }                                                                                                 \
+            const uint32_t mn2 = m + 1 + kDepth;                                                              \
+            if (mn2 < ntile) {                                                                                \
+_Pragma("unroll")                                                                                             \
+                for (int i = 0; i < kPer; ++i) {                                                              \
+                    const uint32_t e = tid + i * 64 * NW;                                                     \
+                    if (e < (uint32_t)kU4) ring[slot][i] = gbase[(uint64_t)mn2 * kU4 + e];                    \
+                }                                                                                             \
+            }                                                                                                 \
+        }                                                                                                     \
+/* Wave-local list maintenance (no cross-wave synchronisation). */                                            \
+        ++since_share;                                                                                        \
+        if (appended || since_share >= 64) {                                                                  \
+            WaveSync::sync();                                                                                 \
+            bool changed = false;                                                                             \
+            uint64_t need = __ballot(lane < nq && cnt_w[lane] > (uint32_t)(B - kMTile));                      \
+            while (need) {                                                                                    \
+                const uint32_t jj = (uint32_t)__builtin_ctzll(need);                                          \
+                need &= need - 1;                                                                             \
+                uint32_t cnt = cnt_w[jj];                                                                     \
+                uint64_t tau = tau_w[jj];                                                                     \
+                compact_global<B, WaveSync>(a.lists + (W.slot_begin + q0 + jj) * (uint64_t)B, cnt, tau, a.K,  \
+                                            sbuf);                                                            \
+                if (lane == 0) {                                                                              \
+                    cnt_w[jj] = cnt;                                                                          \
+                    tau_w[jj] = tau;                                                                          \
+                }                                                                                             \
+                changed = true;                                                                               \
+                WaveSync::sync();                                                                             \
+            }                                                                                                 \
+            if (since_share >= 64) {                                                                          \
+                since_share = 0;                                                                              \
+                share_thresholds<WaveSync>(a.gtau, nq, tau_w, qi_of);                                         \
+                changed = true;                                                                               \
+            }                                                                                                 \
+            if (changed) {                                                                                    \
+_Pragma("unroll")                                                                                             \
+                for (int b = 0; b < kQB; ++b)                                                                 \
+                    if (qv[b]) tauf[b] = tau_as_float(tau_w[32 * b + col]);                                   \
+            }                                                                                                 \
+        }                                                                                                     \
+        __syncthreads(); /* tile m + 1 staged; tile m's buffer free for tile m + 2 */                         \
+    }
+#pragma unroll 1
+    for (uint32_t m0 = 0; m0 < ntile; m0 += kDepth) {
+        FBF_TILE_STEP(0)
+        if constexpr (kDepth > 1) FBF_TILE_STEP(1)
+        if constexpr (kDepth > 2) FBF_TILE_STEP(2)
+        if constexpr (kDepth > 3) FBF_TILE_STEP(3)
+        if constexpr (kDepth > 4) FBF_TILE_STEP(4)
+        if constexpr (kDepth > 5) FBF_TILE_STEP(5)
+    }
+tiles_done:
+#undef FBF_TILE_STEP
     // Final cut: lists longer than K are sorted and cut; shorter ones stay
     // unsorted (the merge does not need order).
     WaveSync::sync();

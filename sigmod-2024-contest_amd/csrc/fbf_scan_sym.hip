@@ -28,7 +28,7 @@ __device__ inline int dot_codes(const int4 (&x)[kCodeInt4], const int4 *q) {
 }
 
 template <int B>
-__device__ void scan_sym_item(const ScanSymArgs &a, const WorkItem &W, uint64_t *sbuf,
+__device__ __attribute__((always_inline)) void scan_sym_item(const ScanSymArgs &a, const WorkItem &W, uint64_t *sbuf,
                               uint64_t *s_tau, uint32_t *s_cnt, int *s_qq, int4 *sq, int4 *stile) {
     const uint32_t lane = threadIdx.x;
     const uint32_t nq = W.q_count;
