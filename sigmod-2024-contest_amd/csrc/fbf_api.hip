@@ -749,6 +749,9 @@ extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, cons
     m.out_dists = out_dists_dev;
     m.rerank = exact ? 0 : 1;
     m.key_int = 0;
+    m.key_pos = mfma ? 1 : 0;  // the MFMA scans key candidates by layout position
+    m.ids_lay[0] = ds->lay[0].ids;
+    m.ids_lay[1] = ds->lay[1].ids;
     ProfEvent ev;
     record_begin(ds, exact ? 1 : 2, st, ev);
     HIP_TRY(launch_merge(m, st));

@@ -138,6 +138,8 @@ struct MergeArgs {
     float *out_dists;          // float distances (key_int == 0) or raw uint32 (key_int == 1)
     int rerank;
     int key_int;               // keys carry an integer distance (symmetric int8 codes)
+    int key_pos;               // key low word = (layout << 31) | layout position, not the id
+    const uint32_t *ids_lay[2];// position -> id per layout (key_pos)
 };
 
 hipError_t launch_scan_f32(const ScanArgs &a, int B, hipStream_t st);

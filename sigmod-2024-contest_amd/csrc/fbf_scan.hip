@@ -241,7 +241,8 @@ __global__ __launch_bounds__(64) void merge_kernel(MergeArgs a) {
         for (uint32_t i = lane; i < (uint32_t)B; i += 64) {
             uint64_t nk = kKeyMax;
             if (i < cnt) {
-                const uint32_t id = (uint32_t)list[i];
+                const uint32_t low = (uint32_t)list[i];
+                const uint32_t id = a.key_pos ? a.ids_lay[low >> 31][low & 0x7FFFFFFFu] : low;
                 const float d = canon_dist_row(a.base_rows + (uint64_t)id * kDim, qv);
                 nk = ((uint64_t)f2ord(d) << 32) | id;
             }
