@@ -610,12 +610,13 @@ extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, cons
     // Approximate modes: the first 1/prepass_div of every large range is
     // scanned first (its own launch), so the per-query thresholds it publishes
     // gate the scan of the rest from the first tile on.
-    const uint32_t pdiv = exact ? 1u : (o.prepass_div ? o.prepass_div : 16u);
+    const uint32_t pdiv = exact ? 1u : (o.prepass_div ? o.prepass_div : 8u);
     Plan P;
     uint64_t n_items_a = 0;
     if (pdiv > 1) {
         std::vector<QRange> qa = qr, qb = qr;
         const uint64_t min_prefix = std::max<uint64_t>(64 * 1024, 8ull * K);
+        uint64_t max_prefix = 0;
         for (uint64_t i = 0; i < nq; ++i) {
             const uint64_t len = qr[i].e - qr[i].s;
             const uint64_t pre = std::max<uint64_t>(min_prefix, (len + pdiv - 1) / pdiv);
@@ -625,8 +626,13 @@ extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, cons
             } else {
                 qb[i].s = qb[i].e;  // whole range in the first pass
             }
+            max_prefix = std::max<uint64_t>(max_prefix, qa[i].e - qa[i].s);
         }
-        make_plan(qa, chunk, gmax, P);
+        // One item covers a group's whole prefix, so the threshold it
+        // publishes is the k'-th of the entire prefix.
+        const uint32_t chunk_a = (uint32_t)std::min<uint64_t>(
+            std::max<uint64_t>(chunk, (max_prefix + kRowTile - 1) / kRowTile * kRowTile + kRowTile), 1u << 30);
+        make_plan(qa, chunk_a, gmax, P);
         order_items(P, !mfma);
         n_items_a = P.items.size();
         Plan PB;

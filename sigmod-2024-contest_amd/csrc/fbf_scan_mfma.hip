@@ -543,10 +543,12 @@ _Pragma("unroll")                                                               
 _Pragma("unroll")                                                                                             \
                 for (int j = 1; j < 16; ++j) mn = fminf(mn, sc[j]);                                           \
                 if (__ballot(mn <= tauf[b]) == 0) continue;                                                   \
-/* Rare path: per-lane bit mask of the scores at or below the threshold, */                                   \
-/* then only the set bits are walked.  Keys carry the layout position     */                                  \
-/* (bit 31: layout) instead of the id, so no load sits on this path; the  */                                  \
-/* merge maps positions to ids before the rerank.                         */                                  \
+/* Rare path: a per-lane bit mask of the scores at or below the lane's   */                                   \
+/* threshold; one LDS atomic reserves that many list slots, then each set */                                  \
+/* bit is stored (static indices, no branches on the data).  A score equal */                                 \
+/* to the threshold may be appended needlessly; compaction drops it.  Keys */                                 \
+/* carry the layout position (bit 31: layout) instead of the id, so no     */                                 \
+/* load sits on this path; the merge maps positions to ids.                */                                 \
                 appended = true;                                                                              \
                 const bool partial = __ballot(qv[b] && !(t >= qs[b] && t + kMTile <= qe[b])) != 0;            \
                 const uint32_t qj = 32 * b + col;                                                             \
@@ -563,19 +565,13 @@ _Pragma("unroll")                                                               
                 }                                                                                             \
                 if (pm) {                                                                                     \
                     uint64_t *list = a.lists + (W.slot_begin + q0 + qj) * (uint64_t)B;                        \
-                    const uint64_t tq = tau_w[qj];                                                            \
+                    uint32_t pos = atomicAdd(&cnt_w[qj], (uint32_t)__builtin_popcount(pm));                   \
                     const uint32_t lay_bit = W.layout << 31;                                                  \
-                    while (pm) {                                                                              \
-                        const int j = __builtin_ctz(pm);                                                      \
-                        pm &= pm - 1;                                                                         \
-                        const uint32_t row = t + 8 * (j / 4) + 4 * half + (j % 4);                            \
-                        float v = sc[0];                                                                      \
 _Pragma("unroll")                                                                                             \
-                        for (int jj = 1; jj < 16; ++jj) v = j == jj ? sc[jj] : v;                             \
-                        const uint64_t key = ((uint64_t)f2ord(v) << 32) | lay_bit | row;                      \
-                        if (key < tq) {                                                                       \
-                            const uint32_t pos = atomicAdd(&cnt_w[qj], 1u);                                   \
-                            list[pos] = key;                                                                  \
+                    for (int j = 0; j < 16; ++j) {                                                            \
+                        if (pm & (1u << j)) {                                                                 \
+                            const uint32_t row = t + 8 * (j / 4) + 4 * half + (j % 4);                        \
+                            list[pos++] = ((uint64_t)f2ord(sc[j]) << 32) | lay_bit | row;                     \
                         }                                                                                     \
                     }                                                                                         \
                 }                                                                                             \
