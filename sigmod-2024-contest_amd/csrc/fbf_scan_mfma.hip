@@ -543,26 +543,36 @@ _Pragma("unroll")                                                               
 _Pragma("unroll")                                                                                             \
                 for (int j = 1; j < 16; ++j) mn = fminf(mn, sc[j]);                                           \
                 if (__ballot(mn <= tauf[b]) == 0) continue;                                                   \
-/* Rare path, one value index j at a time: a compare and a ballot per j;  */                                  \
-/* only the lanes whose score j passes do an LDS slot reservation and a   */                                  \
-/* store.  A score equal to the threshold may be appended needlessly;     */                                  \
-/* compaction drops it.  Keys carry the layout position (bit 31: layout)  */                                  \
-/* instead of the id, so no load sits on this path; the merge maps        */                                  \
-/* positions to ids.                                                      */                                  \
+/* Rare path: a per-lane bit mask of the scores at or below the lane's   */                                   \
+/* threshold; one LDS atomic reserves that many list slots, then each set */                                  \
+/* bit is stored (static indices, no branches on the data).  A score equal */                                 \
+/* to the threshold may be appended needlessly; compaction drops it.  Keys */                                 \
+/* carry the layout position (bit 31: layout) instead of the id, so no     */                                 \
+/* load sits on this path; the merge maps positions to ids.                */                                 \
                 appended = true;                                                                              \
                 const bool partial = __ballot(qv[b] && !(t >= qs[b] && t + kMTile <= qe[b])) != 0;            \
                 const uint32_t qj = 32 * b + col;                                                             \
-                uint64_t *list = a.lists + (W.slot_begin + q0 + qj) * (uint64_t)B;                            \
-                const uint32_t lay_bit = W.layout << 31;                                                      \
+                uint32_t pm = 0;                                                                              \
 _Pragma("unroll")                                                                                             \
-                for (int j = 0; j < 16; ++j) {                                                                \
-                    const uint32_t row = t + 8 * (j / 4) + 4 * half + (j % 4);                                \
-                    bool ok = qv[b] && sc[j] <= tauf[b];                                                      \
-                    if (partial) ok = ok && row >= qs[b] && row < qe[b];                                      \
-                    if (__ballot(ok) == 0) continue;                                                          \
-                    if (ok) {                                                                                 \
-                        const uint32_t pos = atomicAdd(&cnt_w[qj], 1u);                                       \
-                        list[pos] = ((uint64_t)f2ord(sc[j]) << 32) | lay_bit | row;                           \
+                for (int j = 0; j < 16; ++j) pm |= (sc[j] <= tauf[b] ? 1u : 0u) << j;                         \
+                if (!qv[b]) pm = 0;                                                                           \
+                if (partial) {                                                                                \
+_Pragma("unroll")                                                                                             \
+                    for (int j = 0; j < 16; ++j) {                                                            \
+                        const uint32_t row = t + 8 * (j / 4) + 4 * half + (j % 4);                            \
+                        if (row < qs[b] || row >= qe[b]) pm &= ~(1u << j);                                    \
+                    }                                                                                         \
+                }                                                                                             \
+                if (pm) {                                                                                     \
+                    uint64_t *list = a.lists + (W.slot_begin + q0 + qj) * (uint64_t)B;                        \
+                    uint32_t pos = atomicAdd(&cnt_w[qj], (uint32_t)__builtin_popcount(pm));                   \
+                    const uint32_t lay_bit = W.layout << 31;                                                  \
+_Pragma("unroll")                                                                                             \
+                    for (int j = 0; j < 16; ++j) {                                                            \
+                        if (pm & (1u << j)) {                                                                 \
+                            const uint32_t row = t + 8 * (j / 4) + 4 * half + (j % 4);                        \
+                            list[pos++] = ((uint64_t)f2ord(sc[j]) << 32) | lay_bit | row;                     \
+                        }                                                                                     \
                     }                                                                                         \
                 }                                                                                             \
             }                                                                                                 \
