@@ -45,6 +45,7 @@ def parse():
     p.add_argument("--group-queries", type=int, default=0)
     p.add_argument("--persistent", action="store_true")
     p.add_argument("--prepass-div", type=int, default=0)
+    p.add_argument("--prepass-margin", type=int, default=0)
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--recall-sample", type=int, default=16, help="queries checked against FP64")
@@ -83,7 +84,7 @@ def main() -> int:
     out = (torch.empty((nq, a.k), dtype=torch.int32, device=dev),
            torch.empty((nq, a.k), dtype=torch.float32, device=dev))
     kw = dict(k=a.k, mode=mode, kprime=a.kprime, chunk_rows=a.chunk_rows, group_queries=a.group_queries,
-              persistent=a.persistent, out=out, prepass_div=a.prepass_div)
+              persistent=a.persistent, out=out, prepass_div=a.prepass_div, prepass_margin=a.prepass_margin)
 
     for i in range(a.warmup):
         ds.search(qd, metas, **kw)
@@ -99,6 +100,7 @@ def main() -> int:
     ds.profile(False)
     elapsed = t_end - t_start
     stats = ds.last_stats()
+    rescans = ds.last_rescans()
     n_scan, ms_scan = ds.profile_collect("scan")
     n_merge, ms_merge = ds.profile_collect("merge")
     n_rr, ms_rr = ds.profile_collect("rerank")
@@ -109,7 +111,8 @@ def main() -> int:
     qps = nq * a.steps / elapsed
     log(f"{a.steps} steps in {elapsed:.3f}s: {qps:.0f} QPS; scan {ms_scan / max(1, a.steps):.1f} ms/step "
         f"({n_scan} launches), merge+rerank {ms_merge / max(1, a.steps):.1f} ms/step, "
-        f"{stats['work_items']} work items, {stats['pairs_computed'] / max(1, stats['pairs']):.3f}x pairs computed")
+        f"{stats['work_items']} work items, {stats['pairs_computed'] / max(1, stats['pairs']):.3f}x pairs computed, "
+        f"{rescans} queries rescanned")
 
     ids = out[0].cpu().numpy().view(np.uint32)
     dists = out[1].cpu().numpy()

@@ -70,10 +70,16 @@ typedef struct fbf_search_opts {
     uint32_t kprime;      /* 0 -> 4k */
     uint32_t chunk_rows;  /* 0 -> default */
     uint32_t group_queries;
-    uint32_t persistent;  /* 1 -> one persistent work-queue launch for the scan */
-    uint32_t prepass_div; /* approximate modes: scan the first 1/prepass_div of each large range
-                             first to seed the per-query thresholds (0 -> default 16, 1 -> off) */
-    uint32_t reserved[2];
+    uint32_t persistent;  /* scan launch form: 0 -> default (MFMA modes: persistent; others: one
+                             workgroup per work item), 1 -> persistent (a device-filling grid
+                             that pulls work items from queues; the MFMA scans keep one
+                             chunk-major run of items per XCD), 2 -> one workgroup per item */
+    uint32_t prepass_div; /* MFMA modes: a prefix pass over the first 1/prepass_div of each large
+                             range seeds the per-query thresholds (0 -> default 8, 1 -> off) */
+    uint32_t prepass_margin; /* percent of k'/prepass_div keys the prefix pass keeps; its last
+                                kept key is the seed (0 -> default 200).  Queries whose seed proves
+                                too tight are rescanned, so results never depend on it. */
+    uint32_t reserved[1];
 } fbf_search_opts;
 
 /* Create a device-resident dataset on the current HIP device from host
@@ -167,6 +173,12 @@ fbf_status fbf_profile_collect(fbf_dataset *ds, const char *kernel, uint64_t *la
  * (including tile padding), and the number of work items. */
 fbf_status fbf_last_search_stats(const fbf_dataset *ds, uint64_t *pairs, uint64_t *pairs_computed,
                                  uint64_t *work_items);
+
+/* Number of queries of the last fbf_search on this dataset whose seeded
+ * threshold failed verification and were rescanned over their whole range
+ * (0 when the search ran no prefix pass).  Waits for that search's verify
+ * step to finish. */
+fbf_status fbf_last_search_rescans(fbf_dataset *ds, uint64_t *queries);
 
 /* Text of the last error on the calling thread ("" when none). */
 const char *fbf_last_error(void);

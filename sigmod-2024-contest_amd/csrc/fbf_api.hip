@@ -106,13 +106,18 @@ struct ProfEvent {
 // Per-call scratch: the uploaded plan, the per-(item, query) candidate lists
 // and the pinned staging buffer the plan is copied through.
 struct Scratch {
-    DevBuf d_items, d_iq, d_slot_off, d_qslots, d_lists, d_counts, d_queue, d_q, d_out_ids, d_out_d, d_gtau;
-    HostPinned h_stage;
+    DevBuf d_items, d_iq, d_slot_off, d_qslots, d_lists, d_counts, d_queue, d_q, d_out_ids, d_out_d, d_gtau,
+        d_tau0, d_flags, d_pre_end;
+    HostPinned h_stage, h_nflag;
+    hipEvent_t ev_nflag = nullptr;  // recorded after the rescan count is copied to h_nflag
     void release() {
         for (DevBuf *b : {&d_items, &d_iq, &d_slot_off, &d_qslots, &d_lists, &d_counts, &d_queue,
-                          &d_q, &d_out_ids, &d_out_d, &d_gtau})
+                          &d_q, &d_out_ids, &d_out_d, &d_gtau, &d_tau0, &d_flags, &d_pre_end})
             b->release();
         h_stage.release();
+        h_nflag.release();
+        if (ev_nflag) (void)hipEventDestroy(ev_nflag);
+        ev_nflag = nullptr;
     }
 };
 
@@ -136,6 +141,7 @@ struct fbf_dataset {
     bool profile = false;
     std::vector<ProfEvent> events;
     uint64_t last_pairs = 0, last_pairs_computed = 0, last_items = 0;
+    bool last_seeded = false;  // the last search ran the seeded prefix pass (h_nflag is live)
 };
 
 // ---------------------------------------------------------------- helpers
@@ -388,6 +394,7 @@ struct Plan {
     std::vector<ItemQuery> iq;
     std::vector<uint64_t> slot_off;  // [nq + 1]
     std::vector<uint64_t> q_slots;
+    std::vector<uint64_t> pre_end;   // [nq] after concat_plans: end of each query's first-plan slots
     uint64_t pairs = 0, pairs_computed = 0;
 };
 
@@ -498,8 +505,10 @@ static void concat_plans(Plan &A, const Plan &B, uint64_t nq) {
     A.iq.insert(A.iq.end(), B.iq.begin(), B.iq.end());
     std::vector<uint64_t> off(nq + 1, 0), qs;
     qs.reserve(slotsA + B.q_slots.size());
+    A.pre_end.assign(nq, 0);
     for (uint64_t i = 0; i < nq; ++i) {
         for (uint64_t k = A.slot_off[i]; k < A.slot_off[i + 1]; ++k) qs.push_back(A.q_slots[k]);
+        A.pre_end[i] = qs.size();
         for (uint64_t k = B.slot_off[i]; k < B.slot_off[i + 1]; ++k) qs.push_back(B.q_slots[k] + slotsA);
         off[i + 1] = qs.size();
     }
@@ -519,13 +528,15 @@ static fbf_status upload_plan(Scratch &S, const Plan &P, uint64_t nq, int B, hip
     HIP_TRY(S.d_qslots.reserve(std::max<size_t>(1, n_slots) * sizeof(uint64_t)));
     HIP_TRY(S.d_lists.reserve(std::max<size_t>(1, n_slots) * (size_t)B * sizeof(uint64_t)));
     HIP_TRY(S.d_counts.reserve(std::max<size_t>(1, n_slots) * sizeof(uint32_t)));
-    HIP_TRY(S.d_queue.reserve(64));
+    HIP_TRY(S.d_queue.reserve(8 * 64));
     const size_t b_items = P.items.size() * sizeof(WorkItem);
     const size_t b_iq = P.iq.size() * sizeof(ItemQuery);
     const size_t b_off = (nq + 1) * sizeof(uint64_t);
     const size_t b_qs = n_slots * sizeof(uint64_t);
+    const size_t b_pe = P.pre_end.size() * sizeof(uint64_t);
+    if (b_pe) HIP_TRY(S.d_pre_end.reserve(b_pe));
     HIP_TRY(hipStreamSynchronize(st));  // the staging buffer may still feed a previous call
-    HIP_TRY(S.h_stage.reserve(b_items + b_iq + b_off + b_qs + 64));
+    HIP_TRY(S.h_stage.reserve(b_items + b_iq + b_off + b_qs + b_pe + 64));
     char *h = (char *)S.h_stage.p;
     std::memcpy(h, P.items.data(), b_items);
     std::memcpy(h + b_items, P.iq.data(), b_iq);
@@ -535,6 +546,11 @@ static fbf_status upload_plan(Scratch &S, const Plan &P, uint64_t nq, int B, hip
     if (b_iq) HIP_TRY(hipMemcpyAsync(S.d_iq.p, h + b_items, b_iq, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(S.d_slot_off.p, h + b_items + b_iq, b_off, hipMemcpyHostToDevice, st));
     if (b_qs) HIP_TRY(hipMemcpyAsync(S.d_qslots.p, h + b_items + b_iq + b_off, b_qs, hipMemcpyHostToDevice, st));
+    if (b_pe) {
+        char *hp = h + b_items + b_iq + b_off + b_qs;
+        std::memcpy(hp, P.pre_end.data(), b_pe);
+        HIP_TRY(hipMemcpyAsync(S.d_pre_end.p, hp, b_pe, hipMemcpyHostToDevice, st));
+    }
     return FBF_OK;
 }
 
@@ -552,6 +568,8 @@ static MergeArgs merge_args(const Scratch &S, int B, uint64_t nq, uint32_t k, ui
 }
 
 // ---------------------------------------------------------------- search
+// Ranges whose prefix would be shorter than this are scanned in one pass.
+constexpr uint64_t kMinPrefixRows = 16384;
 // Work items in descending cost order (largest first), grouped by layout
 // when each layout is its own launch: big items start early and the small ones
 // fill the tail.
@@ -567,6 +585,35 @@ static void order_items(Plan &P, bool by_layout) {
         if (x.layout != y.layout) return x.layout < y.layout;
         return x.row_begin < y.row_begin;
     });
+}
+
+// Persistent MFMA scans: chunk-major order (layout, first row), so that the
+// items of one XCD's run that are in flight together share their rows in L2.
+static void order_items_chunk_major(Plan &P) {
+    std::stable_sort(P.items.begin(), P.items.end(), [&](const WorkItem &x, const WorkItem &y) {
+        if (x.layout != y.layout) return x.layout < y.layout;
+        if (x.row_begin != y.row_begin) return x.row_begin < y.row_begin;
+        return (uint64_t)(x.row_end - x.row_begin) * x.q_count > (uint64_t)(y.row_end - y.row_begin) * y.q_count;
+    });
+}
+
+// Split items [i0, i1) into 8 contiguous runs of about equal cost (rows x
+// queries; equal counts when by_cost is false), one per XCD.
+static void xcd_runs(const std::vector<WorkItem> &items, uint64_t i0, uint64_t i1, bool by_cost, uint32_t xoff[9]) {
+    auto cost = [&](uint64_t i) -> double {
+        return by_cost ? (double)(items[i].row_end - items[i].row_begin) * items[i].q_count : 1.0;
+    };
+    double total = 0;
+    for (uint64_t i = i0; i < i1; ++i) total += cost(i);
+    xoff[0] = 0;
+    double acc = 0;
+    uint64_t i = i0;
+    for (int x = 1; x < 8; ++x) {
+        const double target = total * x / 8;
+        while (i < i1 && acc + 0.5 * cost(i) < target) acc += cost(i++);
+        xoff[x] = (uint32_t)(i - i0);
+    }
+    xoff[8] = (uint32_t)(i1 - i0);
 }
 
 static fbf_status mode_check(const fbf_dataset *ds, uint32_t mode) {
@@ -596,10 +643,20 @@ extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, cons
         if (K < k) return fail(FBF_ERR_INVALID_ARG, "kprime must be >= k");
         if (K > 896) return fail(FBF_ERR_INVALID_ARG, "kprime must be <= 896");
     }
-    const uint32_t chunk = o.chunk_rows ? (o.chunk_rows + kRowTile - 1) / kRowTile * kRowTile : 256u * 1024u;
+    uint32_t chunk = o.chunk_rows ? (o.chunk_rows + kRowTile - 1) / kRowTile * kRowTile : 256u * 1024u;
+    // The MFMA scans stage candidates with a 26-bit row offset in the item.
+    if (mfma) chunk = std::min<uint32_t>(chunk, 1u << 24);
     // MFMA items carry up to 256 queries (4 waves x 64); VALU items up to 16.
     const uint32_t gcap = mfma ? 256u : (uint32_t)kMaxGroup;
     const uint32_t gmax = o.group_queries ? std::min<uint32_t>(o.group_queries, gcap) : gcap;
+    if (o.persistent > 2) return fail(FBF_ERR_INVALID_ARG, "persistent must be 0, 1 or 2");
+    const bool persist = o.persistent == 1 || (o.persistent == 0 && mfma);
+    // Item order: chunk-major runs per XCD for the persistent MFMA scans, else
+    // largest first.
+    auto order = [&](Plan &Q) {
+        if (mfma && persist) order_items_chunk_major(Q);
+        else order_items(Q, !mfma);
+    };
     hipStream_t st = (hipStream_t)stream;
 
     std::lock_guard<std::mutex> lock(ds->mu);
@@ -607,42 +664,53 @@ extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, cons
     std::vector<QRange> qr;
     s = compute_ranges(ds, metas, nq, qr);
     if (s != FBF_OK) return s;
-    // Approximate modes: the first 1/prepass_div of every large range is
-    // scanned first (its own launch), so the per-query thresholds it publishes
-    // gate the scan of the rest from the first tile on.
-    const uint32_t pdiv = exact ? 1u : (o.prepass_div ? o.prepass_div : 8u);
+    // Approximate modes: a prefix pass over the first 1/prepass_div of every
+    // large range keeps only the kpre = margin * k' / prepass_div best keys of
+    // that prefix; its kpre-th key (tau0) estimates the k'-th key of the whole
+    // range and seeds the threshold of the main pass over the rest, so the main
+    // pass appends about margin * k' candidates per query instead of the many
+    // thousands an unseeded scan appends while its lists fill.  The merge
+    // verifies each seeded query (exact when its k'-th merged key is <= tau0)
+    // and the few that fail are rescanned over their whole range.
+    const uint32_t pdiv = mfma ? (o.prepass_div ? o.prepass_div : 8u) : 1u;
+    const uint32_t margin = o.prepass_margin ? o.prepass_margin : 200u;
+    const uint32_t kpre = (uint32_t)std::min<uint64_t>(K, std::max<uint64_t>(1, ((uint64_t)K * margin + 100ull * pdiv - 1) / (100ull * pdiv)));
     Plan P;
     uint64_t n_items_a = 0;
+    bool seeded = false;
     if (pdiv > 1) {
-        std::vector<QRange> qa = qr, qb = qr;
-        const uint64_t min_prefix = std::max<uint64_t>(64 * 1024, 8ull * K);
+        std::vector<QRange> qa = qr, qb = qr;  // prefixes (empty when not split), the rest
+        const uint64_t min_prefix = std::max<uint64_t>(kMinPrefixRows, 8ull * kpre);
         uint64_t max_prefix = 0;
         for (uint64_t i = 0; i < nq; ++i) {
             const uint64_t len = qr[i].e - qr[i].s;
-            const uint64_t pre = std::max<uint64_t>(min_prefix, (len + pdiv - 1) / pdiv);
-            if (len > 2 * pre) {
+            const uint64_t pre = (len + pdiv - 1) / pdiv;
+            if (pre >= min_prefix) {
                 qa[i].e = qr[i].s + (uint32_t)pre;
                 qb[i].s = qa[i].e;
+                max_prefix = std::max<uint64_t>(max_prefix, pre);
             } else {
-                qb[i].s = qb[i].e;  // whole range in the first pass
+                qa[i].e = qa[i].s;
             }
-            max_prefix = std::max<uint64_t>(max_prefix, qa[i].e - qa[i].s);
         }
-        // One item covers a group's whole prefix, so the threshold it
-        // publishes is the k'-th of the entire prefix.
-        const uint32_t chunk_a = (uint32_t)std::min<uint64_t>(
-            std::max<uint64_t>(chunk, (max_prefix + kRowTile - 1) / kRowTile * kRowTile + kRowTile), 1u << 30);
-        make_plan(qa, chunk_a, gmax, P);
-        order_items(P, !mfma);
-        n_items_a = P.items.size();
-        Plan PB;
-        make_plan(qb, chunk, gmax, PB);
-        order_items(PB, !mfma);
-        concat_plans(P, PB, nq);
-    } else {
+        seeded = max_prefix > 0;
+        if (seeded) {
+            // Prefix items are ordinary chunks: the prefix merge takes the
+            // kpre-th key of the union of their lists (the exact kpre-th key
+            // of the whole prefix: every key up to it is in some list).
+            make_plan(qa, chunk, gmax, P);
+            order(P);
+            n_items_a = P.items.size();
+            Plan PB;
+            make_plan(qb, chunk, gmax, PB);
+            order(PB);
+            concat_plans(P, PB, nq);
+        }
+    }
+    if (!seeded) {
         make_plan(qr, chunk, gmax, P);
-        order_items(P, !mfma);
-        n_items_a = P.items.size();
+        order(P);
+        n_items_a = 0;
     }
     ds->last_pairs = P.pairs;
     ds->last_pairs_computed = P.pairs_computed;
@@ -653,11 +721,33 @@ extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, cons
     s = upload_plan(S, P, nq, B, st);
     if (s != FBF_OK) return s;
     uint32_t *queue = nullptr;
-    if (o.persistent) queue = S.d_queue.as<uint32_t>();
+    if (persist) queue = S.d_queue.as<uint32_t>();
     // Per-query thresholds shared by all work items of a query.
     HIP_TRY(S.d_gtau.reserve(nq * sizeof(uint64_t)));
     uint64_t *gtau = S.d_gtau.as<uint64_t>();
     HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(gtau), 0xFFFFFFFFu, 2 * nq, st));
+
+    // Merge (and rerank) of the per-item lists into the outputs.
+    auto make_merge = [&]() {
+        MergeArgs m = merge_args(S, B, nq, k, K);
+        m.queries = queries_dev;
+        m.base_rows = ds->base_rows;
+        m.out_ids = out_ids_dev;
+        m.out_dists = out_dists_dev;
+        m.rerank = exact ? 0 : 1;
+        m.key_int = 0;
+        m.key_pos = mfma ? 1 : 0;  // the MFMA scans key candidates by layout position
+        m.ids_lay[0] = ds->lay[0].ids;
+        m.ids_lay[1] = ds->lay[1].ids;
+        return m;
+    };
+    auto run_merge = [&](MergeArgs &m) -> fbf_status {
+        ProfEvent ev;
+        record_begin(ds, exact ? 1 : 2, st, ev);
+        HIP_TRY(launch_merge(m, st));
+        record_end(ds, st, ev);
+        return FBF_OK;
+    };
 
     // Scan.  Each work item writes every list slot it owns (count and keys),
     // so no clearing is needed.
@@ -679,21 +769,63 @@ extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, cons
         a.lists = S.d_lists.as<uint64_t>();
         a.counts = S.d_counts.as<uint32_t>();
         a.gtau = gtau;
-        a.K = K;
         const int nw = gmax <= 64 ? 1 : gmax <= 128 ? 2 : 4;
-        // Prefix pass, then the rest.
-        const uint64_t bounds[3] = {0, n_items_a, P.items.size()};
-        for (int ph = 0; ph < 2; ++ph) {
-            if (bounds[ph + 1] == bounds[ph]) continue;
-            a.items = S.d_items.as<WorkItem>() + bounds[ph];
-            a.n_items = bounds[ph + 1] - bounds[ph];
+        auto scan = [&](uint64_t i0, uint64_t i1, uint32_t kk, const uint32_t *mask) -> fbf_status {
+            if (i1 == i0) return FBF_OK;
+            a.items = S.d_items.as<WorkItem>() + i0;
+            a.n_items = i1 - i0;
+            a.K = kk;
+            a.qmask = mask;
             a.queue = queue;
-            if (queue) HIP_TRY(launch_fill_u32(queue, 0u, 1, st));
+            if (queue) {
+                xcd_runs(P.items, i0, i1, mask == nullptr, a.xoff);
+                HIP_TRY(launch_fill_u32(queue, 0u, 8 * 16, st));
+            }
             ProfEvent ev;
             record_begin(ds, 0, st, ev);
             HIP_TRY(launch_scan_mfma(variant, a, B, nw, st));
             record_end(ds, st, ev);
+            return FBF_OK;
+        };
+        if (seeded) {
+            // Prefix pass (kpre kept), seed snapshot, main pass (k' kept).
+            HIP_TRY(S.d_tau0.reserve(nq * sizeof(uint64_t)));
+            HIP_TRY(S.d_flags.reserve((nq + 1) * sizeof(uint32_t)));
+            HIP_TRY(launch_fill_u32(S.d_flags.as<uint32_t>(), 0u, nq + 1, st));
+            s = scan(0, n_items_a, kpre, nullptr);
+            if (s != FBF_OK) return s;
+            uint32_t *flags = S.d_flags.as<uint32_t>();
+            MergeArgs m = make_merge();
+            m.tau0 = S.d_tau0.as<uint64_t>();
+            m.flags = flags;
+            m.gtau = gtau;
+            m.nflag = flags + nq;
+            m.pre_end = S.d_pre_end.as<uint64_t>();
+            // Seeds: the kpre-th key of each query's prefix lists.
+            m.pass = 3;
+            m.kprime = kpre;
+            s = run_merge(m);
+            if (s != FBF_OK) return s;
+            m.kprime = K;
+            s = scan(n_items_a, P.items.size(), K, nullptr);
+            if (s != FBF_OK) return s;
+            // Verify each seeded query; rescan the failures over their whole
+            // range (prefix items now keep k' too) and merge those again.
+            m.pass = 1;
+            s = run_merge(m);
+            if (s != FBF_OK) return s;
+            HIP_TRY(S.h_nflag.reserve(sizeof(uint32_t)));
+            if (!S.ev_nflag) HIP_TRY(hipEventCreateWithFlags(&S.ev_nflag, hipEventDisableTiming));
+            HIP_TRY(hipMemcpyAsync(S.h_nflag.p, flags + nq, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipEventRecord(S.ev_nflag, st));
+            s = scan(0, P.items.size(), K, flags);
+            if (s != FBF_OK) return s;
+            m.pass = 2;
+            ds->last_seeded = true;
+            return run_merge(m);
         }
+        s = scan(0, P.items.size(), K, nullptr);
+        if (s != FBF_OK) return s;
     } else {
         int8_t *qcodes = nullptr;
         if (o.mode == FBF_MODE_SQ8SYM) {
@@ -748,21 +880,9 @@ extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, cons
         }
     }
 
-    MergeArgs m = merge_args(S, B, nq, k, K);
-    m.queries = queries_dev;
-    m.base_rows = ds->base_rows;
-    m.out_ids = out_ids_dev;
-    m.out_dists = out_dists_dev;
-    m.rerank = exact ? 0 : 1;
-    m.key_int = 0;
-    m.key_pos = mfma ? 1 : 0;  // the MFMA scans key candidates by layout position
-    m.ids_lay[0] = ds->lay[0].ids;
-    m.ids_lay[1] = ds->lay[1].ids;
-    ProfEvent ev;
-    record_begin(ds, exact ? 1 : 2, st, ev);
-    HIP_TRY(launch_merge(m, st));
-    record_end(ds, st, ev);
-    return FBF_OK;
+    MergeArgs m = make_merge();
+    ds->last_seeded = false;
+    return run_merge(m);
 }
 
 extern "C" fbf_status fbf_search_host(fbf_dataset *ds, const float *queries, const float *metas,
@@ -852,6 +972,16 @@ extern "C" fbf_status fbf_last_search_stats(const fbf_dataset *ds, uint64_t *pai
     if (pairs) *pairs = ds->last_pairs;
     if (pairs_computed) *pairs_computed = ds->last_pairs_computed;
     if (work_items) *work_items = ds->last_items;
+    return FBF_OK;
+}
+
+extern "C" fbf_status fbf_last_search_rescans(fbf_dataset *ds, uint64_t *queries) {
+    if (!ds || !queries) return fail(FBF_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> lock(ds->mu);
+    *queries = 0;
+    if (!ds->last_seeded) return FBF_OK;
+    HIP_TRY(hipEventSynchronize(ds->scr.ev_nflag));
+    *queries = *static_cast<const uint32_t *>(ds->scr.h_nflag.p);
     return FBF_OK;
 }
 

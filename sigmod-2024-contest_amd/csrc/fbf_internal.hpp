@@ -63,9 +63,11 @@ template <class T> __device__ inline cptr<T> as_const(const T *p) { return (cptr
 
 // List capacity for a kept depth K: room for K kept keys plus at least two
 // full tiles of appends.
+// Capacity of a candidate list that keeps K keys: at least 96 slots of slack
+// above K, so a compaction frees room for several tiles of appends.
 inline int list_capacity(int K) {
-    if (K <= 128) return 256;
-    if (K <= 384) return 512;
+    if (K <= 160) return 256;
+    if (K <= 416) return 512;
     return 1024;
 }
 
@@ -120,7 +122,12 @@ struct ScanMfmaArgs {
     uint32_t *counts;
     uint64_t *gtau;          // [nq] shared per-query threshold (key), tightened by every item
     uint32_t K;
+    // Persistent form (queue != null): the items are split into 8 runs, run x
+    // = [xoff[x], xoff[x + 1]), one per XCD; workgroups take items from their
+    // own XCD's run (counter queue[16 x]) and then from the others' runs.
     uint32_t *queue;
+    uint32_t xoff[9];
+    const uint32_t *qmask;   // [nq] or null: when set, only queries with qmask[qi] != 0 are scanned
 };
 
 struct MergeArgs {
@@ -140,6 +147,18 @@ struct MergeArgs {
     int key_int;               // keys carry an integer distance (symmetric int8 codes)
     int key_pos;               // key low word = (layout << 31) | layout position, not the id
     const uint32_t *ids_lay[2];// position -> id per layout (key_pos)
+    // Seeded scans.  pass 0: plain merge.  pass 3 (after the prefix pass):
+    // merge only the slots [slot_off[q], pre_end[q]) to their kprime-th key and
+    // store it as the seed in tau0[q] and gtau[q]; nothing is emitted.  pass 1:
+    // queries whose k'-th merged key lies above their seed tau0[q] are not
+    // emitted; they get flags[q] = 1, gtau[q] = that k'-th key and count in
+    // *nflag.  pass 2: only flagged queries are merged and emitted.
+    int pass;
+    const uint64_t *pre_end;
+    uint64_t *tau0;
+    uint32_t *flags;
+    uint64_t *gtau;
+    uint32_t *nflag;
 };
 
 hipError_t launch_scan_f32(const ScanArgs &a, int B, hipStream_t st);

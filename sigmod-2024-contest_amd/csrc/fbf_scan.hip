@@ -219,9 +219,10 @@ __global__ __launch_bounds__(64) void merge_kernel(MergeArgs a) {
     __shared__ uint64_t list[B];
     const uint64_t q = blockIdx.x;
     const uint32_t lane = threadIdx.x;
+    if (a.pass == 2 && a.flags[q] == 0) return;
     uint32_t cnt = 0;
     uint64_t tau = kKeyMax;
-    const uint64_t s0 = a.slot_off[q], s1 = a.slot_off[q + 1];
+    const uint64_t s0 = a.slot_off[q], s1 = a.pass == 3 ? a.pre_end[q] : a.slot_off[q + 1];
 #pragma unroll 1
     for (uint64_t si = s0; si < s1; ++si) {
         const uint64_t slot = a.q_slots[si];
@@ -235,6 +236,29 @@ __global__ __launch_bounds__(64) void merge_kernel(MergeArgs a) {
         }
     }
     compact_lds<B>(list, cnt, tau, a.kprime);  // sorted, cnt <= kprime
+    if (a.pass == 3) {
+        if (lane == 0) {
+            a.tau0[q] = tau;
+            a.gtau[q] = tau;
+        }
+        return;
+    }
+    // tau is now the k'-th smallest merged key (kKeyMax when fewer).  The
+    // merged lists hold every candidate key at or below the seed tau0, so the
+    // merged top-k' is exact when its k'-th key is at or below tau0.  Otherwise
+    // the query is rescanned with the k'-th merged key as its threshold (the
+    // k'-th key of any subset bounds the k'-th key of the whole candidate set).
+    if (a.pass == 1) {
+        const uint64_t t0 = a.tau0[q];
+        if (t0 != kKeyMax && tau > t0) {
+            if (lane == 0) {
+                a.flags[q] = 1u;
+                a.gtau[q] = tau;
+                atomicAdd(a.nflag, 1u);
+            }
+            return;
+        }
+    }
 
     if (a.rerank) {
         const float *__restrict__ qv = a.queries + q * kDim;

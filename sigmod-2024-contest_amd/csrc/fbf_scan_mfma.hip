@@ -26,6 +26,7 @@
 //     row 8*(j/4) + 4*(l/32) + j%4, column l%32.
 //   v_mfma_scale_f32_32x32x64_f8f6f4 (e4m3): lane l holds row / column l%32,
 //     k = 32*(l/32) + e, e = 0..31; same D layout.
+#include <atomic>
 #include <utility>
 
 #include "fbf_select.hpp"
@@ -424,35 +425,119 @@ __device__ inline float tau_as_float(uint64_t tau) {
 // item's queries [64 w, 64 w + 64) as two 32-query B blocks in VGPRs, its own
 // candidate lists and thresholds, and synchronises with the other waves only
 // at the one barrier per tile that hands the next tile over.
+//
+// Candidate lists.  In the MFMA output tile a query's 32 rows sit in two lanes
+// (l and l + 32, 16 rows each), so each query's list of B keys is split in two
+// halves, one per lane: a lane appends to its own half at its own count (a
+// register), with no atomics and no cross-lane traffic.  When a half could
+// overflow on the next tile, both halves of that query are sorted together and
+// cut to the K smallest keys, split evenly between the halves; the threshold
+// becomes the K-th key.  At the end of the item each list is made contiguous
+// (its second half moved up, or the whole list cut to K).
 constexpr int kQB = 2;  // 32-query blocks per wave
+
+// Bit j (score j of the lane: row r0 + 8 (j / 4) + (j % 4), r0 = tile row +
+// 4 * half) set when that row lies in [qs, qe).
+__device__ __attribute__((always_inline)) inline uint32_t range_bits(uint32_t t, uint32_t half, uint32_t qs, uint32_t qe) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const uint32_t row = t + 8 * (j / 4) + 4 * half + (j % 4);
+        m |= (row >= qs && row < qe ? 1u : 0u) << j;
+    }
+    return m;
+}
+
+// Per-lane select: lanes in m take b, the others a.  Written as the
+// instruction itself: a select tree over an array, written in C, is turned
+// into an indexed load from a stack copy of the array.
+__device__ __attribute__((always_inline)) inline float lane_select(float a, float b, uint64_t m) {
+    float r;
+    __asm__("v_cndmask_b32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(m));
+    return r;
+}
+
+// Store the keys of the set bits of pm (score sc[j], row r0 + 8 (j / 4) +
+// j % 4) to list[cnt++], lowest bit first.  A lane picks its score with a
+// four-level select on the bit index, so the loop runs only as many times as
+// the lane with the most set bits needs.
+__device__ __attribute__((always_inline)) inline void append_bits(uint32_t pm, const float (&sc)[16], uint32_t r0,
+                                                                  uint32_t lay_bit, uint64_t *list, uint32_t &cnt) {
+    while (pm) {
+        const uint32_t j = (uint32_t)__builtin_ctz(pm);
+        pm &= pm - 1;
+        const uint64_t m8 = __ballot(j & 8u), m4 = __ballot(j & 4u), m2 = __ballot(j & 2u), m1 = __ballot(j & 1u);
+        float v8[8], v4[4], v2[2];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v8[i] = lane_select(sc[i], sc[8 + i], m8);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v4[i] = lane_select(v8[i], v8[4 + i], m4);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) v2[i] = lane_select(v4[i], v4[2 + i], m2);
+        const float v = lane_select(v2[0], v2[1], m1);
+        const uint32_t row = r0 + 8 * (j >> 2) + (j & 3u);
+        list[cnt++] = ((uint64_t)f2ord(v) << 32) | lay_bit | row;
+    }
+}
+
+// Sort the two halves of one split list (counts c0, c1) together and keep the
+// K smallest keys, ceil(keep / 2) in the first half and the rest in the second.
+template <int B>
+__device__ inline void compact_split(uint64_t *list, uint32_t &c0, uint32_t &c1, uint64_t &tau, uint32_t K,
+                                     uint64_t *sbuf) {
+    constexpr uint32_t H = B / 2;
+    const uint32_t lane = threadIdx.x & 63;
+    WaveSync::sync();
+#pragma unroll 2
+    for (int i0 = 0; i0 < B; i0 += 64) {
+        const uint32_t i = i0 + lane;
+        const bool in = i < H ? i < c0 : i - H < c1;
+        sbuf[i] = in ? list[i] : kKeyMax;
+    }
+    WaveSync::sync();
+    wave_sort_lds<B, WaveSync>(sbuf);
+    const uint32_t total = c0 + c1;
+    const uint32_t keep = total < K ? total : K;
+    const uint32_t h0 = (keep + 1) / 2;
+#pragma unroll 2
+    for (int i0 = 0; i0 < B; i0 += 64) {
+        const uint32_t i = i0 + lane;
+        if (i < keep) list[i < h0 ? i : H + (i - h0)] = sbuf[i];
+    }
+    tau = keep == K ? uniform_u64(sbuf[K - 1]) : kKeyMax;
+    c0 = h0;
+    c1 = keep - h0;
+    WaveSync::sync();
+}
 
 template <int VAR, int NW, int B>
 __device__ __attribute__((always_inline)) void scan_mfma_item(const ScanMfmaArgs &a, const WorkItem &W, uint4 (*sA)[TileBytes<VAR>::value / 16],
-                               uint64_t *sbuf, uint64_t *s_tau, uint32_t *s_cnt) {
+                               uint64_t *sbuf, uint64_t *s_tau) {
     constexpr int kU4 = TileBytes<VAR>::value / 16;
     constexpr int kPer = (kU4 + 64 * NW - 1) / (64 * NW);  // staging loads per thread
+    constexpr uint32_t H = B / 2;                          // keys per list half
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t col = lane & 31, half = lane >> 5;
     const uint32_t nq_all = W.q_count;
     const uint32_t q0 = 64 * wave;                            // this wave's first query of the item
     const uint32_t nq = nq_all > q0 ? (nq_all - q0 < 64 ? nq_all - q0 : 64) : 0;
     uint64_t *tau_w = s_tau + q0;
-    uint32_t *cnt_w = s_cnt + q0;
     const LayoutMfma &L = a.lay[W.layout];
     const uint4 *gA = reinterpret_cast<const uint4 *>(L.tiles);
     auto qi_of = [&](uint32_t j) { return a.iq[W.q_begin + q0 + j].qi; };
+    auto list_of = [&](uint32_t j) { return a.lists + (W.slot_begin + q0 + j) * (uint64_t)B; };
 
     // Start from the thresholds earlier items (or the prefix pass) published.
-    for (uint32_t j = lane; j < nq; j += 64) {
+    for (uint32_t j = lane; j < nq; j += 64)
         tau_w[j] = a.gtau ? __hip_atomic_load(&a.gtau[qi_of(j)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
                           : kKeyMax;
-        cnt_w[j] = 0;
-    }
     WaveSync::sync();
     QFrag<VAR> qf[kQB];
     float tauf[kQB];
     uint32_t qs[kQB], qe[kQB];
     bool qv[kQB];
+    uint32_t cnt[kQB];          // keys in this lane's half of its query's list
+    uint64_t *mylist[kQB];      // this lane's half
 #pragma unroll
     for (int b = 0; b < kQB; ++b) {
         const uint32_t j = 32 * b + col;
@@ -464,9 +549,12 @@ __device__ __attribute__((always_inline)) void scan_mfma_item(const ScanMfmaArgs
             qi = e.qi;
             qs[b] = e.s;
             qe[b] = e.e;
+            if (a.qmask && a.qmask[qi] == 0) qv[b] = false;
         }
         load_query<VAR>(a, qv[b], qi, lane, qf[b]);
         tauf[b] = qv[b] ? tau_as_float(tau_w[j]) : -__builtin_inff();
+        cnt[b] = 0;
+        mylist[b] = list_of(j) + half * H;
     }
 
     // Rows stream through LDS.  kDepth tiles are in flight in registers (each
@@ -482,153 +570,149 @@ __device__ __attribute__((always_inline)) void scan_mfma_item(const ScanMfmaArgs
     const u32x4 *gbase = reinterpret_cast<const u32x4 *>(gA) + (uint64_t)(W.row_begin / kMTile) * kU4;
     u32x4 (*sAv)[kU4] = reinterpret_cast<u32x4 (*)[kU4]>(sA);
     u32x4 ring[kDepth][kPer];
-    // Tile 0 straight to LDS, tiles 1..kDepth into the ring.
-    if (ntile > 0) {
+    // Tile 0 straight to LDS, tiles 1..kDepth into the ring (every item has
+    // at least one tile: the planner emits no empty items).  The ring loads
+    // are unconditional, as in the loop below (indices clamped).
 #pragma unroll
-        for (int i = 0; i < kPer; ++i) {
-            const uint32_t e = tid + i * 64 * NW;
-            if (e < (uint32_t)kU4) sAv[0][e] = gbase[e];
-        }
+    for (int i = 0; i < kPer; ++i) {
+        const uint32_t e = tid + i * 64 * NW;
+        if (e < (uint32_t)kU4) sAv[0][e] = gbase[e];
     }
 #pragma unroll
     for (int d = 0; d < kDepth; ++d) {
-        const uint32_t m = 1 + d;
-        if (m < ntile) {
+        const uint32_t m = min(1u + d, ntile - 1);
 #pragma unroll
-            for (int i = 0; i < kPer; ++i) {
-                const uint32_t e = tid + i * 64 * NW;
-                if (e < (uint32_t)kU4) ring[(1 + d) % kDepth][i] = gbase[(uint64_t)m * kU4 + e];
-            }
+        for (int i = 0; i < kPer; ++i) {
+            const uint32_t e = min(tid + i * 64 * NW, (uint32_t)kU4 - 1);
+            ring[(1 + d) % kDepth][i] = gbase[(uint64_t)m * kU4 + e];
         }
+        // Issue order = tile order, as in the loop, so the waits computed for
+        // the loop's first step match those of its later steps.
+        __builtin_amdgcn_sched_barrier(0);
     }
     __syncthreads();
 
+    const uint32_t lay_bit = W.layout << 31;
     uint32_t since_share = 0;
     // The loop body, once per ring slot with a literal slot number so the
     // ring stays in registers (a runtime index would put it in scratch).
 #define FBF_TILE_STEP(U)                                                                              \
     {                                                                                                 \
-        const uint32_t m = m0 + (U);                                                                          \
-        if (m >= ntile) goto tiles_done;                                                                      \
-        const uint32_t t = W.row_begin + m * kMTile;                                                          \
-        bool hit_any = false;                                                                                 \
-        uint64_t hit[kQB];                                                                                    \
-_Pragma("unroll")                                                                                             \
-        for (int b = 0; b < kQB; ++b) {                                                                       \
-            hit[b] = __ballot(qv[b] && t < qe[b] && t + kMTile > qs[b]);                                      \
-            hit_any |= hit[b] != 0;                                                                           \
-        }                                                                                                     \
-        bool appended = false;                                                                                \
-        if (hit_any) {                                                                                        \
-            RowFrag<VAR> rf;                                                                                  \
-            frag_from_lds<VAR>(sA[m & 1u], lane, rf);                                                         \
-            float xn[16];                                                                                     \
-            if constexpr (VAR == 1) {                                                                         \
-                rf.scale = (int)L.scale[t + col] * 0x01010101;                                                \
-_Pragma("unroll")                                                                                             \
-                for (int g = 0; g < 4; ++g) {                                                                 \
-                    const float4 v = *reinterpret_cast<const float4 *>(L.xnorm + t + 8 * g + 4 * half);       \
-                    xn[4 * g + 0] = v.x; xn[4 * g + 1] = v.y; xn[4 * g + 2] = v.z; xn[4 * g + 3] = v.w;       \
-                }                                                                                             \
-            }                                                                                                 \
-_Pragma("unroll")                                                                                             \
-            for (int b = 0; b < kQB; ++b) {                                                                   \
-                if (hit[b] == 0) continue;                                                                    \
-                f32x16 acc = tile_mma<VAR>(rf, qf[b]);                                                        \
-                float sc[16];                                                                                 \
-_Pragma("unroll")                                                                                             \
-                for (int j = 0; j < 16; ++j) sc[j] = VAR == 1 ? acc[j] + xn[j] : acc[j];                      \
-/* One compare per lane: the smallest of its 16 scores. */                                                    \
-                float mn = sc[0];                                                                             \
-_Pragma("unroll")                                                                                             \
-                for (int j = 1; j < 16; ++j) mn = fminf(mn, sc[j]);                                           \
-                if (__ballot(mn <= tauf[b]) == 0) continue;                                                   \
-/* Rare path: a per-lane bit mask of the scores at or below the lane's   */                                   \
-/* threshold; one LDS atomic reserves that many list slots, then each set */                                  \
-/* bit is stored (static indices, no branches on the data).  A score equal */                                 \
-/* to the threshold may be appended needlessly; compaction drops it.  Keys */                                 \
-/* carry the layout position (bit 31: layout) instead of the id, so no     */                                 \
-/* load sits on this path; the merge maps positions to ids.                */                                 \
-                appended = true;                                                                              \
-                const bool partial = __ballot(qv[b] && !(t >= qs[b] && t + kMTile <= qe[b])) != 0;            \
-                const uint32_t qj = 32 * b + col;                                                             \
-                uint32_t pm = 0;                                                                              \
-_Pragma("unroll")                                                                                             \
-                for (int j = 0; j < 16; ++j) pm |= (sc[j] <= tauf[b] ? 1u : 0u) << j;                         \
-                if (!qv[b]) pm = 0;                                                                           \
-                if (partial) {                                                                                \
-_Pragma("unroll")                                                                                             \
-                    for (int j = 0; j < 16; ++j) {                                                            \
-                        const uint32_t row = t + 8 * (j / 4) + 4 * half + (j % 4);                            \
-                        if (row < qs[b] || row >= qe[b]) pm &= ~(1u << j);                                    \
-                    }                                                                                         \
-                }                                                                                             \
-                if (pm) {                                                                                     \
-                    uint64_t *list = a.lists + (W.slot_begin + q0 + qj) * (uint64_t)B;                        \
-                    uint32_t pos = atomicAdd(&cnt_w[qj], (uint32_t)__builtin_popcount(pm));                   \
-                    const uint32_t lay_bit = W.layout << 31;                                                  \
-_Pragma("unroll")                                                                                             \
-                    for (int j = 0; j < 16; ++j) {                                                            \
-                        if (pm & (1u << j)) {                                                                 \
-                            const uint32_t row = t + 8 * (j / 4) + 4 * half + (j % 4);                        \
-                            list[pos++] = ((uint64_t)f2ord(sc[j]) << 32) | lay_bit | row;                     \
-                        }                                                                                     \
-                    }                                                                                         \
-                }                                                                                             \
-            }                                                                                                 \
-        }                                                                                                     \
-/* Tile m + 1 (issued kDepth tiles ago) goes to LDS; its ring slot */                                         \
-/* takes the loads of tile m + 1 + kDepth. */                                                                 \
-        {                                                                                                     \
-            const int slot = ((U) + 1) % kDepth;                                                              \
-            if (m + 1 < ntile) {                                                                              \
-_Pragma("unroll")                                                                                             \
-                for (int i = 0; i < kPer; ++i) {                                                              \
-                    const uint32_t e = tid + i * 64 * NW;                                                     \
-                    if (e < (uint32_t)kU4) sAv[(m + 1) & 1u][e] = ring[slot][i];                               \
-                }                                                                                             \
-            }                                                                                                 \
-            const uint32_t mn2 = m + 1 + kDepth;                                                              \
-            if (mn2 < ntile) {                                                                                \
-_Pragma("unroll")                                                                                             \
-                for (int i = 0; i < kPer; ++i) {                                                              \
-                    const uint32_t e = tid + i * 64 * NW;                                                     \
-                    if (e < (uint32_t)kU4) ring[slot][i] = gbase[(uint64_t)mn2 * kU4 + e];                    \
-                }                                                                                             \
-            }                                                                                                 \
-        }                                                                                                     \
-/* Wave-local list maintenance (no cross-wave synchronisation). */                                            \
-        ++since_share;                                                                                        \
-        if (appended || since_share >= 64) {                                                                  \
-            WaveSync::sync();                                                                                 \
-            bool changed = false;                                                                             \
-            uint64_t need = __ballot(lane < nq && cnt_w[lane] > (uint32_t)(B - kMTile));                      \
-            while (need) {                                                                                    \
-                const uint32_t jj = (uint32_t)__builtin_ctzll(need);                                          \
-                need &= need - 1;                                                                             \
-                uint32_t cnt = cnt_w[jj];                                                                     \
-                uint64_t tau = tau_w[jj];                                                                     \
-                compact_global<B, WaveSync>(a.lists + (W.slot_begin + q0 + jj) * (uint64_t)B, cnt, tau, a.K,  \
-                                            sbuf);                                                            \
-                if (lane == 0) {                                                                              \
-                    cnt_w[jj] = cnt;                                                                          \
-                    tau_w[jj] = tau;                                                                          \
-                }                                                                                             \
-                changed = true;                                                                               \
-                WaveSync::sync();                                                                             \
-            }                                                                                                 \
-            if (since_share >= 64) {                                                                          \
-                since_share = 0;                                                                              \
-                share_thresholds<WaveSync>(a.gtau, nq, tau_w, qi_of);                                         \
-                changed = true;                                                                               \
-            }                                                                                                 \
-            if (changed) {                                                                                    \
-_Pragma("unroll")                                                                                             \
-                for (int b = 0; b < kQB; ++b)                                                                 \
-                    if (qv[b]) tauf[b] = tau_as_float(tau_w[32 * b + col]);                                   \
-            }                                                                                                 \
-        }                                                                                                     \
-        __syncthreads(); /* tile m + 1 staged; tile m's buffer free for tile m + 2 */                         \
+        const uint32_t m = m0 + (U);                                                                  \
+/* Steps past the last tile (the last pass of the loop runs all kDepth  */                            \
+/* steps) compute nothing but keep staging, so that every pass issues    */                           \
+/* the same loads in the same order and the loop has no exit in between. */                           \
+        const bool live = m < ntile;                                                                  \
+        const uint32_t t = W.row_begin + m * kMTile;                                                  \
+        bool hit_any = false;                                                                         \
+        uint64_t hit[kQB];                                                                            \
+_Pragma("unroll")                                                                                     \
+        for (int b = 0; b < kQB; ++b) {                                                               \
+            hit[b] = __ballot(live && qv[b] && t < qe[b] && t + kMTile > qs[b]);                      \
+            hit_any |= hit[b] != 0;                                                                   \
+        }                                                                                             \
+        bool appended = false;                                                                        \
+        if (hit_any) {                                                                                \
+            RowFrag<VAR> rf;                                                                          \
+            frag_from_lds<VAR>(sA[m & 1u], lane, rf);                                                 \
+            float xn[16];                                                                             \
+            if constexpr (VAR == 1) {                                                                 \
+                rf.scale = (int)L.scale[t + col] * 0x01010101;                                        \
+_Pragma("unroll")                                                                                     \
+                for (int g = 0; g < 4; ++g) {                                                         \
+                    const float4 v = *reinterpret_cast<const float4 *>(L.xnorm + t + 8 * g + 4 * half); \
+                    xn[4 * g + 0] = v.x; xn[4 * g + 1] = v.y; xn[4 * g + 2] = v.z; xn[4 * g + 3] = v.w; \
+                }                                                                                     \
+            }                                                                                         \
+_Pragma("unroll")                                                                                     \
+            for (int b = 0; b < kQB; ++b) {                                                           \
+                if (hit[b] == 0) continue;                                                            \
+                f32x16 acc = tile_mma<VAR>(rf, qf[b]);                                                \
+                float sc[16];                                                                         \
+_Pragma("unroll")                                                                                     \
+                for (int j = 0; j < 16; ++j) sc[j] = VAR == 1 ? acc[j] + xn[j] : acc[j];              \
+/* One compare per lane: the smallest of its 16 scores. */                                            \
+                float mn = sc[0];                                                                     \
+_Pragma("unroll")                                                                                     \
+                for (int j = 1; j < 16; ++j) mn = fminf(mn, sc[j]);                                   \
+                if (__ballot(mn <= tauf[b]) == 0) continue;                                           \
+/* Rare path: a per-lane bit mask of the scores at or below the lane's  */                           \
+/* threshold (inside its query's range on a partial tile); each lane     */                           \
+/* then stores its set bits to its half of the list, lowest first.  A    */                           \
+/* score equal to the threshold may be stored needlessly; compaction     */                           \
+/* drops it.  Keys carry the layout position (bit 31: layout), not the   */                           \
+/* id; the merge maps positions to ids.                                  */                           \
+                appended = true;                                                                      \
+                uint32_t pm = 0;                                                                      \
+_Pragma("unroll")                                                                                     \
+                for (int j = 0; j < 16; ++j) pm |= (sc[j] <= tauf[b] ? 1u : 0u) << j;                 \
+                if (__ballot(qv[b] && !(t >= qs[b] && t + kMTile <= qe[b]))) pm &= range_bits(t, half, qs[b], qe[b]); \
+                append_bits(pm, sc, t + 4 * half, lay_bit, mylist[b], cnt[b]);                        \
+            }                                                                                         \
+        }                                                                                             \
+/* Tile m + 1 (issued kDepth tiles ago) goes to LDS; its ring slot */                                 \
+/* takes the loads of tile m + 1 + kDepth. */                                                         \
+        {                                                                                             \
+            const int slot = ((U) + 1) % kDepth;                                                      \
+            if (m + 1 < ntile) {                                                                      \
+_Pragma("unroll")                                                                                     \
+                for (int i = 0; i < kPer; ++i) {                                                      \
+                    const uint32_t e = tid + i * 64 * NW;                                             \
+                    if (e < (uint32_t)kU4) sAv[(m + 1) & 1u][e] = ring[slot][i];                      \
+                }                                                                                     \
+            }                                                                                         \
+/* Every thread issues the same number of loads on every step (tile and */                            \
+/* piece indices clamped: the extra loads re-read valid bytes and are    */                           \
+/* never used), so the compiler's wait before the LDS store above counts */                           \
+/* only this slot's loads and leaves the later tiles' loads in flight.   */                           \
+            const uint32_t mn2 = min(m + 1 + kDepth, ntile - 1);                                      \
+_Pragma("unroll")                                                                                     \
+            for (int i = 0; i < kPer; ++i) {                                                          \
+                const uint32_t e = min(tid + i * 64 * NW, (uint32_t)kU4 - 1);                         \
+                ring[slot][i] = gbase[(uint64_t)mn2 * kU4 + e];                                       \
+            }                                                                                         \
+        }                                                                                             \
+/* Wave-local list maintenance (no cross-wave synchronisation): halves */                             \
+/* that the next tile could overflow (16 keys per lane and block) are   */                            \
+/* compacted with their other halves; every 64 tiles the thresholds    */                            \
+/* are shared with the other items of the same queries.                */                             \
+        since_share += live ? 1u : 0u;                                                                \
+        if (appended || since_share >= 64) {                                                          \
+            bool changed = false;                                                                     \
+_Pragma("unroll 1")                                                                                   \
+            for (int b = 0; b < kQB; ++b) {                                                           \
+                const uint32_t cb = b ? cnt[1] : cnt[0];                                              \
+                const uint64_t nb = __ballot(cb > H - 16);                                            \
+                uint32_t need = (uint32_t)nb | (uint32_t)(nb >> 32);                                  \
+                while (need) {                                                                        \
+                    const uint32_t c = (uint32_t)__builtin_ctz(need);                                 \
+                    need &= need - 1;                                                                 \
+                    const uint32_t jj = 32 * b + c;                                                   \
+                    uint32_t c0 = __builtin_amdgcn_readlane(cb, c);                                   \
+                    uint32_t c1 = __builtin_amdgcn_readlane(cb, 32 + c);                              \
+                    uint64_t tau = tau_w[jj];                                                         \
+                    compact_split<B>(list_of(jj), c0, c1, tau, a.K, sbuf);                            \
+                    if (col == c) {                                                                   \
+                        if (b) cnt[1] = half ? c1 : c0;                                               \
+                        else cnt[0] = half ? c1 : c0;                                                 \
+                    }                                                                                 \
+                    if (lane == 0) tau_w[jj] = tau;                                                   \
+                    changed = true;                                                                   \
+                }                                                                                     \
+            }                                                                                         \
+            if (since_share >= 64) {                                                                  \
+                since_share = 0;                                                                      \
+                share_thresholds<WaveSync>(a.gtau, nq, tau_w, qi_of);                                 \
+                changed = true;                                                                       \
+            }                                                                                         \
+            if (changed) {                                                                            \
+                WaveSync::sync();                                                                     \
+_Pragma("unroll")                                                                                     \
+                for (int b = 0; b < kQB; ++b)                                                         \
+                    if (qv[b]) tauf[b] = tau_as_float(tau_w[32 * b + col]);                           \
+            }                                                                                         \
+        }                                                                                             \
+        __syncthreads(); /* tile m + 1 staged; tile m's buffer free for tile m + 2 */                 \
     }
 #pragma unroll 1
     for (uint32_t m0 = 0; m0 < ntile; m0 += kDepth) {
@@ -639,21 +723,47 @@ _Pragma("unroll")                                                               
         if constexpr (kDepth > 4) FBF_TILE_STEP(4)
         if constexpr (kDepth > 5) FBF_TILE_STEP(5)
     }
-tiles_done:
 #undef FBF_TILE_STEP
-    // Final cut: lists longer than K are sorted and cut; shorter ones stay
-    // unsorted (the merge does not need order).
+    // Final cut: each list becomes contiguous, [0, count): longer than K ->
+    // sorted and cut to K; otherwise its second half moves up behind the
+    // first (unsorted: the merge does not need order).
     WaveSync::sync();
     for (uint32_t j = 0; j < nq; ++j) {
-        uint32_t cnt = cnt_w[j];
-        uint64_t tau = tau_w[j];
-        if (cnt > a.K) {
-            compact_global<B, WaveSync>(a.lists + (W.slot_begin + q0 + j) * (uint64_t)B, cnt, tau, a.K, sbuf);
+        const uint32_t b = j >> 5, c = j & 31;
+        const uint32_t cb = b ? cnt[1] : cnt[0];
+        uint32_t c0 = __builtin_amdgcn_readlane(cb, c), c1 = __builtin_amdgcn_readlane(cb, 32 + c);
+        uint64_t *list = list_of(j);
+        uint32_t count = c0 + c1;
+        if (count > a.K) {
+            uint64_t tau = tau_w[j];
+            compact_split<B>(list, c0, c1, tau, a.K, sbuf);
             if (lane == 0) tau_w[j] = tau;
+            count = c0 + c1;
         }
-        if (lane == 0) a.counts[W.slot_begin + q0 + j] = cnt;
+        if (c1) {
+            // [H, H + c1) -> [c0, c0 + c1); c0 + c1 <= K <= H, so the ranges are disjoint.
+            WaveSync::sync();
+            for (uint32_t i = lane; i < c1; i += 64) list[c0 + i] = list[H + i];
+        }
+        if (lane == 0 && (a.qmask == nullptr || a.qmask[qi_of(j)] != 0)) a.counts[W.slot_begin + q0 + j] = count;
     }
     share_thresholds<WaveSync>(a.gtau, nq, tau_w, qi_of);
+}
+
+// Masked launches: does any query of the item take part?  (Uniform over the
+// workgroup, so a skipped item returns from every wave together.)
+template <int NW>
+__device__ inline bool item_active(const ScanMfmaArgs &a, const WorkItem &W) {
+    if (a.qmask == nullptr) return true;
+    int any = 0;
+    for (uint32_t j = threadIdx.x; j < W.q_count; j += 64 * NW) any |= a.qmask[a.iq[W.q_begin + j].qi] != 0;
+    return __syncthreads_or(any) != 0;
+}
+
+// XCD (XCC) the calling wave runs on: hardware register XCC_ID (id 20),
+// bits [3:0].
+__device__ inline uint32_t xcc_id() {
+    return __builtin_amdgcn_s_getreg((20) | (0 << 6) | ((4 - 1) << 11)) & 0xFu;
 }
 
 template <int VAR, int NW, int B>
@@ -662,22 +772,42 @@ void scan_mfma_kernel(ScanMfmaArgs a) {
     __shared__ uint4 sA[2][TileBytes<VAR>::value / 16];
     __shared__ uint64_t sbuf[NW][B];
     __shared__ uint64_t s_tau[64 * NW];
-    __shared__ uint32_t s_cnt[64 * NW];
     __shared__ uint32_t s_idx;
     uint64_t *my_sbuf = sbuf[threadIdx.x >> 6];
-    if (a.queue == nullptr) {
-        const WorkItem W = as_const(a.items)[blockIdx.x];
-        scan_mfma_item<VAR, NW, B>(a, W, sA, my_sbuf, s_tau, s_cnt);
-        return;
-    }
-    for (;;) {
-        if (threadIdx.x == 0) s_idx = atomicAdd(a.queue, 1u);
+    // One workgroup per item (queue == null: item blockIdx.x), or the
+    // persistent form.  Workgroups are dealt to the 8 XCDs round-robin, and
+    // each XCD has its own L2: a persistent workgroup drains its XCD's run of
+    // items first (the runs are chunk-major, so the workgroups of one XCD
+    // stream the same rows for different query groups at about the same time
+    // and share them in L2), then helps the other runs.  One call site keeps
+    // a single inlined copy of the item code.
+    const uint32_t me = xcc_id();
+    for (uint32_t round = 0;; ++round) {
+        if (threadIdx.x == 0) {
+            uint32_t idx = kPadId;
+            if (a.queue == nullptr) {
+                idx = round == 0 ? blockIdx.x : kPadId;
+            } else {
+                for (uint32_t t = 0; t < 8; ++t) {
+                    const uint32_t x = (me + t) & 7u;
+                    const uint32_t n = a.xoff[x + 1] - a.xoff[x];
+                    if (n == 0) continue;
+                    if (__hip_atomic_load(&a.queue[16 * x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= n) continue;
+                    const uint32_t i = atomicAdd(&a.queue[16 * x], 1u);
+                    if (i < n) {
+                        idx = a.xoff[x] + i;
+                        break;
+                    }
+                }
+            }
+            s_idx = idx;
+        }
         __syncthreads();
         const uint32_t idx = __builtin_amdgcn_readfirstlane(s_idx);
         __syncthreads();
-        if (idx >= a.n_items) break;
+        if (idx == kPadId) break;
         const WorkItem W = as_const(a.items)[idx];
-        scan_mfma_item<VAR, NW, B>(a, W, sA, my_sbuf, s_tau, s_cnt);
+        if (item_active<NW>(a, W)) scan_mfma_item<VAR, NW, B>(a, W, sA, my_sbuf, s_tau);
         __syncthreads();
     }
 }
@@ -700,6 +830,34 @@ static hipError_t launch_mfma_var(const ScanMfmaArgs &a, int B, int nw, uint64_t
     return launch_mfma_nw<VAR, 4>(a, B, grid, st);
 }
 
+template <int VAR, int NW>
+static int occupancy_nw(int B) {
+    int n = 0;
+    hipError_t e = hipErrorInvalidValue;
+    if (B == 256) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, scan_mfma_kernel<VAR, NW, 256>, 64 * NW, 0);
+    if (B == 512) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, scan_mfma_kernel<VAR, NW, 512>, 64 * NW, 0);
+    if (B == 1024) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, scan_mfma_kernel<VAR, NW, 1024>, 64 * NW, 0);
+    return (e == hipSuccess && n > 0) ? n : 8 / NW;
+}
+
+template <int VAR>
+static int occupancy_var(int B, int nw) {
+    return nw == 1 ? occupancy_nw<VAR, 1>(B) : nw == 2 ? occupancy_nw<VAR, 2>(B) : occupancy_nw<VAR, 4>(B);
+}
+
+// Resident workgroups per CU of a scan instantiation (cached per variant, B, nw).
+static int persistent_per_cu(int variant, int B, int nw) {
+    static std::atomic<int> cache[3][3][3] = {};
+    const int bi = B == 256 ? 0 : B == 512 ? 1 : 2, wi = nw == 1 ? 0 : nw == 2 ? 1 : 2;
+    std::atomic<int> &c = cache[variant][bi][wi];
+    int v = c.load(std::memory_order_relaxed);
+    if (v == 0) {
+        v = variant == 0 ? occupancy_var<0>(B, nw) : variant == 1 ? occupancy_var<1>(B, nw) : occupancy_var<2>(B, nw);
+        c.store(v, std::memory_order_relaxed);
+    }
+    return v;
+}
+
 // nw: waves per workgroup (1, 2 or 4); an item carries at most 64 * nw queries.
 hipError_t launch_scan_mfma(int variant, const ScanMfmaArgs &a, int B, int nw, hipStream_t st) {
     if (a.n_items == 0) return hipSuccess;
@@ -709,7 +867,11 @@ hipError_t launch_scan_mfma(int variant, const ScanMfmaArgs &a, int B, int nw, h
         int dev = 0, cus = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        grid = std::min<uint64_t>((uint64_t)cus * (8 / nw), a.n_items);
+        // As many workgroups as fit on the device at once (a multiple of the 8
+        // XCDs, so each XCD holds the same number).
+        const int per_cu = persistent_per_cu(variant, B, nw);
+        grid = std::min<uint64_t>((uint64_t)cus * per_cu, a.n_items);
+        if (grid > 8) grid = grid / 8 * 8;
     }
     if (variant == 0) return launch_mfma_var<0>(a, B, nw, grid, st);
     if (variant == 1) return launch_mfma_var<1>(a, B, nw, grid, st);

@@ -37,8 +37,8 @@ EXPORTS = [
     "fbf_dataset_create", "fbf_dataset_create_device", "fbf_dataset_destroy", "fbf_filter_count",
     "fbf_search", "fbf_search_host", "fbf_dataset_quant_params", "fbf_register_codes",
     "fbf_unregister_codes", "fbf_bruteforce", "fbf_bruteforce_subgraph", "fbf_bruteforce_batch",
-    "fbf_profile_enable", "fbf_profile_collect", "fbf_last_search_stats", "fbf_last_error",
-    "fbf_version",
+    "fbf_profile_enable", "fbf_profile_collect", "fbf_last_search_stats", "fbf_last_search_rescans",
+    "fbf_last_error", "fbf_version",
 ]
 
 
@@ -49,7 +49,7 @@ class FbfError(RuntimeError):
 class SearchOpts(C.Structure):
     _fields_ = [("mode", C.c_uint32), ("kprime", C.c_uint32), ("chunk_rows", C.c_uint32),
                 ("group_queries", C.c_uint32), ("persistent", C.c_uint32), ("prepass_div", C.c_uint32),
-                ("reserved", C.c_uint32 * 2)]
+                ("prepass_margin", C.c_uint32), ("reserved", C.c_uint32 * 1)]
 
 
 _lib = None
@@ -84,6 +84,7 @@ def lib() -> C.CDLL:
     L.fbf_profile_collect.argtypes = [_P, C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]
     L.fbf_last_search_stats.argtypes = [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                         C.POINTER(C.c_uint64)]
+    L.fbf_last_search_rescans.argtypes = [_P, C.POINTER(C.c_uint64)]
     for name in EXPORTS:
         if name not in ("fbf_last_error", "fbf_version"):
             getattr(L, name).restype = C.c_int
@@ -152,19 +153,21 @@ class Dataset:
         return c.value, s.value
 
     @staticmethod
-    def _opts(mode, kprime, chunk_rows, group_queries, persistent, prepass_div=0) -> SearchOpts:
+    def _opts(mode, kprime, chunk_rows, group_queries, persistent, prepass_div=0,
+              prepass_margin=0) -> SearchOpts:
         o = SearchOpts()
         o.mode = MODES[mode] if isinstance(mode, str) else int(mode)
         o.kprime = kprime
         o.chunk_rows = chunk_rows
         o.group_queries = group_queries
-        o.persistent = int(persistent)
+        o.persistent = int(persistent)  # 0 default, 1 (True) persistent, 2 one workgroup per item
         o.prepass_div = prepass_div
+        o.prepass_margin = prepass_margin
         return o
 
     def search(self, queries, metas: np.ndarray, k: int = 100, mode="exact", kprime: int = 0,
                chunk_rows: int = 0, group_queries: int = 0, persistent: bool = False,
-               out=None, stream=None, prepass_div: int = 0):
+               out=None, stream=None, prepass_div: int = 0, prepass_margin: int = 0):
         """queries: torch CUDA float32 [nq, d]; metas: numpy float32 [nq, 4].
         Returns torch (ids int32 view of uint32 [nq, k], dists float32 [nq, k])
         on the queries' device; asynchronous on ``stream`` (torch stream or None
@@ -179,21 +182,21 @@ class Dataset:
         else:
             ids, dists = out
         st = (stream or torch.cuda.current_stream(queries.device)).cuda_stream
-        o = self._opts(mode, kprime, chunk_rows, group_queries, persistent, prepass_div)
+        o = self._opts(mode, kprime, chunk_rows, group_queries, persistent, prepass_div, prepass_margin)
         _check(lib().fbf_search(self.h, _P(_dev_ptr(queries)), _np_ptr(metas), nq, k, C.byref(o),
                                 _P(_dev_ptr(ids)), _P(_dev_ptr(dists)), _P(st)), "fbf_search")
         return ids, dists
 
     def search_host(self, queries: np.ndarray, metas: np.ndarray, k: int = 100, mode="exact",
                     kprime: int = 0, chunk_rows: int = 0, group_queries: int = 0,
-                    persistent: bool = False, prepass_div: int = 0):
+                    persistent: bool = False, prepass_div: int = 0, prepass_margin: int = 0):
         """Host-memory form: returns numpy (ids uint32 [nq, k], dists float32 [nq, k])."""
         queries = np.ascontiguousarray(queries, np.float32)
         metas = np.ascontiguousarray(metas, np.float32)
         nq = queries.shape[0]
         ids = np.zeros((nq, k), np.uint32)
         dists = np.zeros((nq, k), np.float32)
-        o = self._opts(mode, kprime, chunk_rows, group_queries, persistent, prepass_div)
+        o = self._opts(mode, kprime, chunk_rows, group_queries, persistent, prepass_div, prepass_margin)
         _check(lib().fbf_search_host(self.h, _np_ptr(queries), _np_ptr(metas), nq, k, C.byref(o),
                                      _np_ptr(ids), _np_ptr(dists)), "fbf_search_host")
         return ids, dists
@@ -220,6 +223,12 @@ class Dataset:
         _check(lib().fbf_last_search_stats(self.h, C.byref(p), C.byref(pc), C.byref(wi)),
                "fbf_last_search_stats")
         return {"pairs": p.value, "pairs_computed": pc.value, "work_items": wi.value}
+
+    def last_rescans(self) -> int:
+        """Queries of the last search rescanned after their seed failed."""
+        n = C.c_uint64(0)
+        _check(lib().fbf_last_search_rescans(self.h, C.byref(n)), "fbf_last_search_rescans")
+        return n.value
 
 
 # ---------------------------------------------------------------- reference-shaped calls

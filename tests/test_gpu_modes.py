@@ -83,6 +83,10 @@ def test_mode_small_kprime_and_chunks(pkg, orc, c0, mode):
     vecs, labels, ts, q, metas, ds = c0
     ids, d = _search(ds, q[:64], metas[:64], 10, mode, kprime=40, chunk_rows=128, group_queries=7,
                      persistent=True)
+    # The top-k' by approximate key is unique: the other launch form returns the same.
+    ids2, d2 = _search(ds, q[:64], metas[:64], 10, mode, kprime=40, chunk_rows=128, group_queries=7,
+                       persistent=2)
+    assert np.array_equal(ids, ids2) and np.array_equal(d, d2)
     eid, ed = orc.exact_predicate(vecs, labels.astype(np.int32), ts, q[:64], metas[:64], k=10, mode="canon")
     rec, _ = orc.recall(ids, eid)
     assert rec >= RECALL_FLOOR[mode] - 0.05
