@@ -36,7 +36,7 @@ def parse():
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--config", type=int, default=1)
-    p.add_argument("--mode", default="exact", choices=["exact", "sq8", "bf16", "fp8", "sq8sym"])
+    p.add_argument("--mode", default="bf16", choices=["exact", "sq8", "bf16", "fp8", "sq8sym"])
     p.add_argument("--kprime", type=int, default=0)
     p.add_argument("--k", type=int, default=100)
     p.add_argument("--n", type=int, default=0, help="override base rows (smaller runs)")
@@ -104,6 +104,7 @@ def main() -> int:
     n_scan, ms_scan = ds.profile_collect("scan")
     n_merge, ms_merge = ds.profile_collect("merge")
     n_rr, ms_rr = ds.profile_collect("rerank")
+    n_plan, ms_plan = ds.profile_collect("plan")
     n_merge += n_rr
     ms_merge += ms_rr
     ds.profile_collect("all")
@@ -112,7 +113,7 @@ def main() -> int:
     log(f"{a.steps} steps in {elapsed:.3f}s: {qps:.0f} QPS; scan {ms_scan / max(1, a.steps):.1f} ms/step "
         f"({n_scan} launches), merge+rerank {ms_merge / max(1, a.steps):.1f} ms/step, "
         f"{stats['work_items']} work items, {stats['pairs_computed'] / max(1, stats['pairs']):.3f}x pairs computed, "
-        f"{rescans} queries rescanned")
+        f"{rescans} queries rescanned, host planning {ms_plan / max(1, a.steps):.1f} ms/step")
 
     ids = out[0].cpu().numpy().view(np.uint32)
     dists = out[1].cpu().numpy()

@@ -63,11 +63,12 @@ typedef struct fbf_dataset fbf_dataset;
 
 /* Search tuning.  Zero fields take the library defaults.  kprime is the
  * over-fetch depth of the approximate modes (ignored by FBF_MODE_EXACT_F32);
- * chunk_rows is the rows per work item; group_queries the queries sharing one
- * streamed tile (<= 16 for the VALU scans, <= 256 for the MFMA scans). */
+ * chunk_rows is the rows per work item (default 262144); group_queries the
+ * queries sharing one streamed tile (<= 16 for the VALU scans, default 16;
+ * <= 256 for the MFMA scans, default 128). */
 typedef struct fbf_search_opts {
     uint32_t mode;        /* fbf_mode */
-    uint32_t kprime;      /* 0 -> 4k */
+    uint32_t kprime;      /* 0 -> 3k/2 (BF16, SQ8) or 4k (FP8, SQ8SYM), at most 896 */
     uint32_t chunk_rows;  /* 0 -> default */
     uint32_t group_queries;
     uint32_t persistent;  /* scan launch form: 0 -> default (MFMA modes: persistent; others: one
@@ -163,7 +164,9 @@ fbf_status fbf_bruteforce_batch(const char *codes_base, const uint64_t *starts,
 /* When enabled, every kernel launch of this dataset is bracketed by HIP
  * events on its stream; fbf_profile_collect waits for them and returns the
  * launch count and summed milliseconds of the kernel named `kernel`
- * ("scan", "merge", "rerank", "build", or "all"), then clears the log. */
+ * ("scan", "merge", "rerank", "build", or "all"), then clears the log.
+ * "plan" returns the host time fbf_search spent planning the work items and
+ * queueing their upload (one entry per search). */
 fbf_status fbf_profile_enable(fbf_dataset *ds, int enable);
 fbf_status fbf_profile_collect(fbf_dataset *ds, const char *kernel, uint64_t *launches,
                                double *total_ms);

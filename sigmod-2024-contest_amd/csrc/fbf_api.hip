@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <thread>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -99,22 +100,31 @@ struct LayoutDev {
 };
 
 struct ProfEvent {
-    int kind;  // 0 scan, 1 merge, 2 rerank(part of merge), 3 build
-    hipEvent_t a, b;
+    int kind;  // 0 scan, 1 merge, 2 rerank(part of merge), 3 build, 4 plan (host)
+    hipEvent_t a = nullptr, b = nullptr;
+    double host_ms = 0.0;  // kind 4: wall time on the host
 };
 
 // Per-call scratch: the uploaded plan, the per-(item, query) candidate lists
 // and the pinned staging buffer the plan is copied through.
 struct Scratch {
-    DevBuf d_items, d_iq, d_slot_off, d_qslots, d_lists, d_counts, d_queue, d_q, d_out_ids, d_out_d, d_gtau,
+    DevBuf d_items, d_iq, d_slot_off, d_lists, d_counts, d_queue, d_q, d_out_ids, d_out_d, d_gtau,
         d_tau0, d_flags, d_pre_end;
-    HostPinned h_stage, h_nflag;
+    HostPinned h_stage[2], h_nflag;
+    hipEvent_t ev_stage[2] = {nullptr, nullptr};  // recorded after the copies out of h_stage[i]
+    int stage_next = 0;
     hipEvent_t ev_nflag = nullptr;  // recorded after the rescan count is copied to h_nflag
+    hipStream_t last_stream = nullptr;
+    bool used = false;
     void release() {
-        for (DevBuf *b : {&d_items, &d_iq, &d_slot_off, &d_qslots, &d_lists, &d_counts, &d_queue,
+        for (DevBuf *b : {&d_items, &d_iq, &d_slot_off, &d_lists, &d_counts, &d_queue,
                           &d_q, &d_out_ids, &d_out_d, &d_gtau, &d_tau0, &d_flags, &d_pre_end})
             b->release();
-        h_stage.release();
+        for (int i = 0; i < 2; ++i) {
+            h_stage[i].release();
+            if (ev_stage[i]) (void)hipEventDestroy(ev_stage[i]);
+            ev_stage[i] = nullptr;
+        }
         h_nflag.release();
         if (ev_nflag) (void)hipEventDestroy(ev_nflag);
         ev_nflag = nullptr;
@@ -371,8 +381,8 @@ extern "C" fbf_status fbf_dataset_destroy(fbf_dataset *ds) {
         if (p) (void)hipFree(p);
     ds->scr.release();
     for (auto &ev : ds->events) {
-        hipEventDestroy(ev.a);
-        hipEventDestroy(ev.b);
+        if (ev.a) hipEventDestroy(ev.a);
+        if (ev.b) hipEventDestroy(ev.b);
     }
     delete ds;
     return FBF_OK;
@@ -392,26 +402,62 @@ extern "C" fbf_status fbf_filter_count(const fbf_dataset *ds, const float *meta,
 struct Plan {
     std::vector<WorkItem> items;
     std::vector<ItemQuery> iq;
-    std::vector<uint64_t> slot_off;  // [nq + 1]
-    std::vector<uint64_t> q_slots;
-    std::vector<uint64_t> pre_end;   // [nq] after concat_plans: end of each query's first-plan slots
+    std::vector<uint64_t> slot_off;  // [nq + 1]: query q's lists are slots [slot_off[q], slot_off[q + 1])
+    std::vector<uint64_t> pre_end;   // [nq], two-pass plans: end of each query's first-pass slots
+    std::vector<uint32_t> nchunk;    // [nq] chunks of the query's group (slots the query owns)
     uint64_t pairs = 0, pairs_computed = 0;
 };
 
+// Run fn(begin, end) over [0, n) in up to 16 host threads (one when n is small).
+template <class Fn>
+static void parallel_ranges(uint64_t n, uint64_t min_per_thread, Fn fn) {
+    const uint64_t hw = std::max(1u, std::thread::hardware_concurrency());
+    const uint64_t nt = std::min<uint64_t>({16, hw, (n + min_per_thread - 1) / min_per_thread});
+    if (nt <= 1) {
+        fn(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    const uint64_t per = (n + nt - 1) / nt;
+    for (uint64_t t = 0; t < nt; ++t) {
+        const uint64_t b = t * per, e = std::min(n, b + per);
+        if (b < e) th.emplace_back(fn, b, e);
+    }
+    for (auto &x : th) x.join();
+}
+
+// Each query's candidate range (binary searches over the sorted layouts; the
+// batch is split across host threads).
 static fbf_status compute_ranges(const fbf_dataset *ds, const float *metas, uint64_t nq,
                                  std::vector<QRange> &qr) {
     qr.resize(nq);
-    for (uint64_t i = 0; i < nq; ++i) {
-        if (!query_range(ds, metas + 4 * i, qr[i]))
-            return fail(FBF_ERR_INVALID_ARG, "query " + std::to_string(i) + ": bad filter type");
-        qr[i].qi = (uint32_t)i;
-    }
+    std::atomic<uint64_t> bad{UINT64_MAX};
+    parallel_ranges(nq, 4096, [&](uint64_t b, uint64_t e) {
+        for (uint64_t i = b; i < e; ++i) {
+            if (!query_range(ds, metas + 4 * i, qr[i])) {
+                uint64_t cur = bad.load();
+                while (i < cur && !bad.compare_exchange_weak(cur, i)) {
+                }
+            }
+            qr[i].qi = (uint32_t)i;
+        }
+    });
+    if (bad.load() != UINT64_MAX)
+        return fail(FBF_ERR_INVALID_ARG, "query " + std::to_string(bad.load()) + ": bad filter type");
     return FBF_OK;
 }
 
 // Group queries and cut their spans into work items.  qr[i].qi must be i.
+// Work items from query ranges.  Queries of one layout are sorted by range
+// and packed greedily into groups of at most group_max whose shared span
+// exceeds the longest member's range by at most 25%; a group's span is cut
+// into chunks of chunk_rows, one work item each, and every item carries all
+// members of its group (a member whose range misses the chunk costs one empty
+// list).  Query q owns nchunk[q] consecutive list slots, one per chunk of its
+// group; finish_slots numbers them.  Host work is O(nq log nq + items).
 static void make_plan(const std::vector<QRange> &qr, uint32_t chunk_rows, uint32_t group_max, Plan &P) {
     const uint64_t nq = qr.size();
+    P.nchunk.assign(nq, 0);
     for (const QRange &r : qr) P.pairs += r.e - r.s;
     std::vector<uint32_t> order;
     order.reserve(nq);
@@ -425,45 +471,36 @@ static void make_plan(const std::vector<QRange> &qr, uint32_t chunk_rows, uint32
         return a < b;
     });
 
-    std::vector<uint64_t> per_q_slots(nq, 0);
-    std::vector<std::pair<uint32_t, uint64_t>> slot_owner;  // (query, slot)
-    uint64_t slot_next = 0;
-
-    auto emit_group = [&](const std::vector<uint32_t> &g) {
-        uint32_t us = UINT32_MAX, ue = 0;
-        for (uint32_t qi : g) {
-            us = std::min(us, qr[qi].s);
-            ue = std::max(ue, qr[qi].e);
-        }
+    auto emit_group = [&](const uint32_t *g, uint32_t gn, uint32_t us, uint32_t ue) {
         const uint32_t b0 = us / kRowTile * kRowTile;
-        for (uint64_t cb = b0; cb < ue; cb += chunk_rows) {
-            const uint32_t ce = (uint32_t)std::min<uint64_t>(cb + chunk_rows, ue);
+        const uint32_t nc = (uint32_t)((ue - b0 + (uint64_t)chunk_rows - 1) / chunk_rows);
+        const uint32_t q_begin = (uint32_t)P.iq.size();
+        for (uint32_t k = 0; k < gn; ++k) {
+            const QRange &r = qr[g[k]];
+            P.iq.push_back(ItemQuery{g[k], r.s, r.e, 0});
+            P.nchunk[g[k]] = nc;
+            // Rows streamed for this query: the chunks its range touches.
+            const uint64_t c_lo = (r.s - b0) / chunk_rows, c_hi = (r.e - 1 - b0) / chunk_rows;
+            const uint64_t lo = b0 + c_lo * chunk_rows, hi = std::min<uint64_t>(b0 + (c_hi + 1) * chunk_rows, ue);
+            P.pairs_computed += (hi - lo + kRowTile - 1) / kRowTile * kRowTile;
+        }
+        for (uint32_t c = 0; c < nc; ++c) {
             WorkItem W{};
-            W.row_begin = (uint32_t)cb;
-            W.row_end = ce;
-            W.q_begin = (uint32_t)P.iq.size();
+            W.row_begin = b0 + c * chunk_rows;
+            W.row_end = (uint32_t)std::min<uint64_t>((uint64_t)W.row_begin + chunk_rows, ue);
+            W.q_begin = q_begin;
+            W.q_count = gn;
             W.layout = qr[g[0]].layout;
-            W.slot_begin = slot_next;
-            for (uint32_t qi : g) {
-                if (qr[qi].s < ce && qr[qi].e > cb) {
-                    P.iq.push_back(ItemQuery{qi, qr[qi].s, qr[qi].e, 0});
-                    slot_owner.emplace_back(qi, slot_next++);
-                    per_q_slots[qi]++;
-                }
-            }
-            W.q_count = (uint32_t)(P.iq.size() - W.q_begin);
-            if (W.q_count == 0) continue;
-            const uint64_t tiles = (ce - cb + kRowTile - 1) / kRowTile;
-            P.pairs_computed += tiles * kRowTile * W.q_count;
+            W.chunk = c;
             P.items.push_back(W);
         }
     };
 
-    std::vector<uint32_t> g;
+    size_t g0 = 0;  // current group: order[g0, i)
     uint32_t g_layout = 0, g_s = 0, g_e = 0, g_maxlen = 0;
-    for (uint32_t qi : order) {
-        const QRange &r = qr[qi];
-        bool fits = !g.empty() && g.size() < group_max && r.layout == g_layout;
+    for (size_t i = 0; i < order.size(); ++i) {
+        const QRange &r = qr[order[i]];
+        bool fits = i > g0 && i - g0 < group_max && r.layout == g_layout;
         if (fits) {
             const uint32_t ns = std::min(g_s, r.s), ne = std::max(g_e, r.e);
             const uint32_t ml = std::max(g_maxlen, r.e - r.s);
@@ -471,93 +508,94 @@ static void make_plan(const std::vector<QRange> &qr, uint32_t chunk_rows, uint32
             fits = (uint64_t)(ne - ns) * 4 <= (uint64_t)ml * 5 + 4 * kRowTile;
         }
         if (!fits) {
-            if (!g.empty()) emit_group(g);
-            g.clear();
+            if (i > g0) emit_group(order.data() + g0, (uint32_t)(i - g0), g_s, g_e);
+            g0 = i;
             g_layout = r.layout;
             g_s = r.s;
             g_e = r.e;
             g_maxlen = 0;
         }
-        g.push_back(qi);
         g_s = std::min(g_s, r.s);
         g_e = std::max(g_e, r.e);
         g_maxlen = std::max(g_maxlen, r.e - r.s);
     }
-    if (!g.empty()) emit_group(g);
-
-    P.slot_off.assign(nq + 1, 0);
-    for (uint64_t i = 0; i < nq; ++i) P.slot_off[i + 1] = P.slot_off[i] + per_q_slots[i];
-    P.q_slots.assign(slot_next, 0);
-    std::vector<uint64_t> fill(P.slot_off.begin(), P.slot_off.end() - 1);
-    for (auto &so : slot_owner) P.q_slots[fill[so.first]++] = so.second;
+    if (order.size() > g0) emit_group(order.data() + g0, (uint32_t)(order.size() - g0), g_s, g_e);
 }
 
-// Append plan B (same batch) after plan A: B's items follow A's, its slots
-// are renumbered after A's, and each query's slot list holds A's then B's.
-static void concat_plans(Plan &A, const Plan &B, uint64_t nq) {
-    const uint64_t slotsA = A.q_slots.size();
+// Number the list slots of a one-pass plan: query q owns nchunk[q] slots from
+// slot_off[q]; item W's list of member j is slot iq[q_begin + j].slot_base + W.chunk.
+static fbf_status finish_slots(Plan &P, uint64_t nq) {
+    P.slot_off.assign(nq + 1, 0);
+    for (uint64_t i = 0; i < nq; ++i) P.slot_off[i + 1] = P.slot_off[i] + P.nchunk[i];
+    if (P.slot_off[nq] >= 0xFFFFFFFFull) return fail(FBF_ERR_INVALID_ARG, "batch needs more than 2^32 candidate lists");
+    for (ItemQuery &e : P.iq) e.slot_base = (uint32_t)P.slot_off[e.qi];
+    return FBF_OK;
+}
+
+// Append plan Bp (the second pass) to plan A (the first pass) and number the
+// slots: query q owns A's chunks, then B's, from slot_off[q]; pre_end[q] ends
+// its first-pass slots.
+static fbf_status concat_plans(Plan &A, const Plan &Bp, uint64_t nq) {
     const uint32_t iqA = (uint32_t)A.iq.size();
-    for (WorkItem W : B.items) {
-        W.slot_begin += slotsA;
+    for (WorkItem W : Bp.items) {
         W.q_begin += iqA;
         A.items.push_back(W);
     }
-    A.iq.insert(A.iq.end(), B.iq.begin(), B.iq.end());
-    std::vector<uint64_t> off(nq + 1, 0), qs;
-    qs.reserve(slotsA + B.q_slots.size());
+    A.iq.insert(A.iq.end(), Bp.iq.begin(), Bp.iq.end());
+    A.slot_off.assign(nq + 1, 0);
     A.pre_end.assign(nq, 0);
     for (uint64_t i = 0; i < nq; ++i) {
-        for (uint64_t k = A.slot_off[i]; k < A.slot_off[i + 1]; ++k) qs.push_back(A.q_slots[k]);
-        A.pre_end[i] = qs.size();
-        for (uint64_t k = B.slot_off[i]; k < B.slot_off[i + 1]; ++k) qs.push_back(B.q_slots[k] + slotsA);
-        off[i + 1] = qs.size();
+        A.pre_end[i] = A.slot_off[i] + A.nchunk[i];
+        A.slot_off[i + 1] = A.pre_end[i] + Bp.nchunk[i];
     }
-    A.slot_off.swap(off);
-    A.q_slots.swap(qs);
-    A.pairs += B.pairs;
-    A.pairs_computed += B.pairs_computed;
+    if (A.slot_off[nq] >= 0xFFFFFFFFull) return fail(FBF_ERR_INVALID_ARG, "batch needs more than 2^32 candidate lists");
+    for (size_t k = 0; k < A.iq.size(); ++k) {
+        ItemQuery &e = A.iq[k];
+        e.slot_base = (uint32_t)(k < iqA ? A.slot_off[e.qi] : A.pre_end[e.qi]);
+    }
+    A.pairs += Bp.pairs;
+    A.pairs_computed += Bp.pairs_computed;
+    return FBF_OK;
 }
 
 // Reserve scratch for a plan with list capacity B and copy the plan to the
 // device through pinned memory (asynchronous on st).
 static fbf_status upload_plan(Scratch &S, const Plan &P, uint64_t nq, int B, hipStream_t st) {
-    const uint64_t n_slots = P.q_slots.size();
+    const uint64_t n_slots = P.slot_off.empty() ? 0 : P.slot_off[nq];
     HIP_TRY(S.d_items.reserve(std::max<size_t>(1, P.items.size()) * sizeof(WorkItem)));
     HIP_TRY(S.d_iq.reserve(std::max<size_t>(1, P.iq.size()) * sizeof(ItemQuery)));
     HIP_TRY(S.d_slot_off.reserve((nq + 1) * sizeof(uint64_t)));
-    HIP_TRY(S.d_qslots.reserve(std::max<size_t>(1, n_slots) * sizeof(uint64_t)));
     HIP_TRY(S.d_lists.reserve(std::max<size_t>(1, n_slots) * (size_t)B * sizeof(uint64_t)));
     HIP_TRY(S.d_counts.reserve(std::max<size_t>(1, n_slots) * sizeof(uint32_t)));
     HIP_TRY(S.d_queue.reserve(8 * 64));
     const size_t b_items = P.items.size() * sizeof(WorkItem);
     const size_t b_iq = P.iq.size() * sizeof(ItemQuery);
     const size_t b_off = (nq + 1) * sizeof(uint64_t);
-    const size_t b_qs = n_slots * sizeof(uint64_t);
     const size_t b_pe = P.pre_end.size() * sizeof(uint64_t);
     if (b_pe) HIP_TRY(S.d_pre_end.reserve(b_pe));
-    HIP_TRY(hipStreamSynchronize(st));  // the staging buffer may still feed a previous call
-    HIP_TRY(S.h_stage.reserve(b_items + b_iq + b_off + b_qs + b_pe + 64));
-    char *h = (char *)S.h_stage.p;
+    // Two pinned staging buffers used in turn: a buffer is refilled only after
+    // the copies that read it (queued by the search before last) are done.
+    const int sb = S.stage_next;
+    S.stage_next ^= 1;
+    if (S.ev_stage[sb]) HIP_TRY(hipEventSynchronize(S.ev_stage[sb]));
+    else HIP_TRY(hipEventCreateWithFlags(&S.ev_stage[sb], hipEventDisableTiming));
+    HIP_TRY(S.h_stage[sb].reserve(b_items + b_iq + b_off + b_pe + 64));
+    char *h = (char *)S.h_stage[sb].p;
     std::memcpy(h, P.items.data(), b_items);
     std::memcpy(h + b_items, P.iq.data(), b_iq);
     std::memcpy(h + b_items + b_iq, P.slot_off.data(), b_off);
-    std::memcpy(h + b_items + b_iq + b_off, P.q_slots.data(), b_qs);
+    if (b_pe) std::memcpy(h + b_items + b_iq + b_off, P.pre_end.data(), b_pe);
     if (b_items) HIP_TRY(hipMemcpyAsync(S.d_items.p, h, b_items, hipMemcpyHostToDevice, st));
     if (b_iq) HIP_TRY(hipMemcpyAsync(S.d_iq.p, h + b_items, b_iq, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(S.d_slot_off.p, h + b_items + b_iq, b_off, hipMemcpyHostToDevice, st));
-    if (b_qs) HIP_TRY(hipMemcpyAsync(S.d_qslots.p, h + b_items + b_iq + b_off, b_qs, hipMemcpyHostToDevice, st));
-    if (b_pe) {
-        char *hp = h + b_items + b_iq + b_off + b_qs;
-        std::memcpy(hp, P.pre_end.data(), b_pe);
-        HIP_TRY(hipMemcpyAsync(S.d_pre_end.p, hp, b_pe, hipMemcpyHostToDevice, st));
-    }
+    if (b_pe) HIP_TRY(hipMemcpyAsync(S.d_pre_end.p, h + b_items + b_iq + b_off, b_pe, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipEventRecord(S.ev_stage[sb], st));
     return FBF_OK;
 }
 
 static MergeArgs merge_args(const Scratch &S, int B, uint64_t nq, uint32_t k, uint32_t kprime) {
     MergeArgs m{};
     m.slot_off = S.d_slot_off.as<uint64_t>();
-    m.q_slots = S.d_qslots.as<uint64_t>();
     m.lists = S.d_lists.as<uint64_t>();
     m.counts = S.d_counts.as<uint32_t>();
     m.list_cap = (uint32_t)B;
@@ -639,16 +677,21 @@ extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, cons
     // Kept depth per list: k for the exact scan, k' (over-fetch) otherwise.
     uint32_t K = k;
     if (!exact) {
-        K = o.kprime ? o.kprime : 4 * k;
+        // Default over-fetch: 3k/2 for the BF16 and SQ8 scans (their score
+        // errors are small against the gaps between neighbours), 4k for FP8
+        // and the symmetric int8 codes.
+        const bool fine = o.mode == FBF_MODE_BF16 || o.mode == FBF_MODE_SQ8;
+        K = o.kprime ? o.kprime : std::min<uint32_t>(fine ? (3 * k + 1) / 2 : 4 * k, 896);
         if (K < k) return fail(FBF_ERR_INVALID_ARG, "kprime must be >= k");
         if (K > 896) return fail(FBF_ERR_INVALID_ARG, "kprime must be <= 896");
     }
     uint32_t chunk = o.chunk_rows ? (o.chunk_rows + kRowTile - 1) / kRowTile * kRowTile : 256u * 1024u;
     // The MFMA scans stage candidates with a 26-bit row offset in the item.
     if (mfma) chunk = std::min<uint32_t>(chunk, 1u << 24);
-    // MFMA items carry up to 256 queries (4 waves x 64); VALU items up to 16.
+    // MFMA items carry up to 256 queries (4 waves x 64), 128 by default (two
+    // waves share each row tile); VALU items up to 16.
     const uint32_t gcap = mfma ? 256u : (uint32_t)kMaxGroup;
-    const uint32_t gmax = o.group_queries ? std::min<uint32_t>(o.group_queries, gcap) : gcap;
+    const uint32_t gmax = o.group_queries ? std::min<uint32_t>(o.group_queries, gcap) : mfma ? 128u : gcap;
     if (o.persistent > 2) return fail(FBF_ERR_INVALID_ARG, "persistent must be 0, 1 or 2");
     const bool persist = o.persistent == 1 || (o.persistent == 0 && mfma);
     // Item order: chunk-major runs per XCD for the persistent MFMA scans, else
@@ -661,6 +704,12 @@ extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, cons
 
     std::lock_guard<std::mutex> lock(ds->mu);
     HIP_TRY(hipSetDevice(ds->device));
+    const auto t_plan0 = std::chrono::steady_clock::now();
+    // The dataset's scratch is reused by every search: a search on another
+    // stream than the previous one first waits for that one.
+    if (ds->scr.used && ds->scr.last_stream != st) HIP_TRY(hipStreamSynchronize(ds->scr.last_stream));
+    ds->scr.used = true;
+    ds->scr.last_stream = st;
     std::vector<QRange> qr;
     s = compute_ranges(ds, metas, nq, qr);
     if (s != FBF_OK) return s;
@@ -698,28 +747,43 @@ extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, cons
             // Prefix items are ordinary chunks: the prefix merge takes the
             // kpre-th key of the union of their lists (the exact kpre-th key
             // of the whole prefix: every key up to it is in some list).
+            Plan PB;
+            std::thread tb([&] {
+                make_plan(qb, chunk, gmax, PB);
+                order(PB);
+            });
             make_plan(qa, chunk, gmax, P);
             order(P);
+            tb.join();
             n_items_a = P.items.size();
-            Plan PB;
-            make_plan(qb, chunk, gmax, PB);
-            order(PB);
-            concat_plans(P, PB, nq);
+            s = concat_plans(P, PB, nq);
+            if (s != FBF_OK) return s;
         }
     }
     if (!seeded) {
         make_plan(qr, chunk, gmax, P);
         order(P);
         n_items_a = 0;
+        s = finish_slots(P, nq);
+        if (s != FBF_OK) return s;
     }
     ds->last_pairs = P.pairs;
     ds->last_pairs_computed = P.pairs_computed;
     ds->last_items = P.items.size();
 
-    const int B = list_capacity((int)K);
+    // MFMA scans: room for about one K of appends between compactions once K
+    // is large (a scan whose threshold is still open then compacts rarely);
+    // each of the two halves a lane appends to keeps room above ceil(K / 2) + 16.
+    const int B = mfma ? (K <= 160 ? 256 : K <= 300 ? 512 : 1024) : list_capacity((int)K);
     Scratch &S = ds->scr;
     s = upload_plan(S, P, nq, B, st);
     if (s != FBF_OK) return s;
+    if (ds->profile) {
+        ProfEvent ev;
+        ev.kind = 4;
+        ev.host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_plan0).count();
+        ds->events.push_back(ev);
+    }
     uint32_t *queue = nullptr;
     if (persist) queue = S.d_queue.as<uint32_t>();
     // Per-query thresholds shared by all work items of a query.
@@ -943,6 +1007,7 @@ extern "C" fbf_status fbf_profile_collect(fbf_dataset *ds, const char *kernel, u
     else if (!std::strcmp(kernel, "merge")) want = 1;
     else if (!std::strcmp(kernel, "rerank")) want = 2;
     else if (!std::strcmp(kernel, "build")) want = 3;
+    else if (!std::strcmp(kernel, "plan")) want = 4;
     else if (std::strcmp(kernel, "all")) return fail(FBF_ERR_INVALID_ARG, "unknown kernel name");
     uint64_t cnt = 0;
     double ms = 0.0;
@@ -950,6 +1015,11 @@ extern "C" fbf_status fbf_profile_collect(fbf_dataset *ds, const char *kernel, u
     for (auto &ev : ds->events) {
         if (want >= 0 && ev.kind != want) {
             keep.push_back(ev);
+            continue;
+        }
+        if (ev.kind == 4) {
+            ms += ev.host_ms;
+            ++cnt;
             continue;
         }
         HIP_TRY(hipEventSynchronize(ev.b));
@@ -1077,6 +1147,8 @@ static fbf_status sym_topk(const int8_t *codes_dev, uint64_t n_rows, const uint6
     }
     Plan P;
     make_plan(qr, 64 * 1024, kMaxGroup, P);
+    fbf_status s0 = finish_slots(P, nq);
+    if (s0 != FBF_OK) return s0;
     const int B = list_capacity((int)k);
     Scratch &S = E.scr;
     fbf_status s = upload_plan(S, P, nq, B, st);

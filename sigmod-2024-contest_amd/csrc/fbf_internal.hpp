@@ -11,28 +11,32 @@ constexpr int kRowTile = 64;     // rows per wave tile: one row per lane
 constexpr uint32_t kPadId = 0xFFFFFFFFu;
 constexpr uint64_t kKeyMax = ~0ull;
 
-// A work item: one wave scans rows [row_begin, row_end) of one sorted layout
-// for up to kMaxGroup queries.  Query j of the item is entry q_begin + j of
-// the plan's per-item query table; its candidate list lives in slot
-// slot_begin + j.
+// A work item: one workgroup scans rows [row_begin, row_end) of one sorted
+// layout for the queries of one group: entries q_begin .. q_begin + q_count - 1
+// of the plan's group table.  The item is chunk number `chunk` of its group's
+// span; query j's candidate list lives in slot iq[q_begin + j].slot_base + chunk.
 struct WorkItem {
     uint32_t row_begin;   // multiple of kRowTile
     uint32_t row_end;
     uint32_t q_begin;
     uint32_t q_count;
     uint32_t layout;      // 0 = (label, ts) order, 1 = ts order
-    uint32_t pad;
-    uint64_t slot_begin;
+    uint32_t chunk;
+    uint32_t pad[2];
 };
 
-// Per (work item, query) entry: the query's index in the batch and its own
-// candidate range in the item's layout.
+// Per (group, query) entry: the query's index in the batch, its candidate
+// range in the group's layout, and its first list slot for this pass.
 struct ItemQuery {
     uint32_t qi;
     uint32_t s;
     uint32_t e;
-    uint32_t pad;
+    uint32_t slot_base;
 };
+
+__device__ __host__ inline uint64_t list_slot(const ItemQuery &e, const WorkItem &W) {
+    return (uint64_t)e.slot_base + W.chunk;
+}
 
 constexpr int kMaxGroup = 16;    // queries per work item
 
@@ -131,8 +135,7 @@ struct ScanMfmaArgs {
 };
 
 struct MergeArgs {
-    const uint64_t *slot_off;  // [nq + 1] CSR over q_slots
-    const uint64_t *q_slots;   // slot indices
+    const uint64_t *slot_off;  // [nq + 1]: query q's lists are slots [slot_off[q], slot_off[q + 1])
     const uint64_t *lists;
     const uint32_t *counts;
     uint32_t list_cap;         // B of the scan lists

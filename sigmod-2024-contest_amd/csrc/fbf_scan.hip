@@ -133,7 +133,7 @@ __device__ __attribute__((always_inline)) void scan_f32_item(const ScanArgs &a, 
             const uint64_t key = ((uint64_t)f2ord(acc) << 32) | id;
             uint64_t tau = s_tau[j];
             uint32_t cnt = s_cnt[j];
-            uint64_t *list = a.lists + (W.slot_begin + j) * (uint64_t)B;
+            uint64_t *list = a.lists + list_slot(e, W) * (uint64_t)B;
             offer_global<B>(list, cnt, tau, a.K, sbuf, in && key < tau, key);
             s_tau[j] = tau;
             s_cnt[j] = cnt;
@@ -143,9 +143,10 @@ __device__ __attribute__((always_inline)) void scan_f32_item(const ScanArgs &a, 
     for (uint32_t j = 0; j < nq; ++j) {
         uint64_t tau = s_tau[j];
         uint32_t cnt = s_cnt[j];
-        uint64_t *list = a.lists + (W.slot_begin + j) * (uint64_t)B;
+        const uint64_t slot = list_slot(iq[j], W);
+        uint64_t *list = a.lists + slot * (uint64_t)B;
         compact_global<B>(list, cnt, tau, a.K, sbuf);
-        if (lane == 0) a.counts[W.slot_begin + j] = cnt;
+        if (lane == 0) a.counts[slot] = cnt;
         s_tau[j] = tau;
     }
     share_thresholds(a.gtau, nq, s_tau, [&](uint32_t j) { return iq[j].qi; });
@@ -225,7 +226,7 @@ __global__ __launch_bounds__(64) void merge_kernel(MergeArgs a) {
     const uint64_t s0 = a.slot_off[q], s1 = a.pass == 3 ? a.pre_end[q] : a.slot_off[q + 1];
 #pragma unroll 1
     for (uint64_t si = s0; si < s1; ++si) {
-        const uint64_t slot = a.q_slots[si];
+        const uint64_t slot = si;
         const uint32_t c = a.counts[slot];
         const uint64_t *L = a.lists + slot * (uint64_t)a.list_cap;
 #pragma unroll 1

@@ -435,6 +435,7 @@ __device__ inline float tau_as_float(uint64_t tau) {
 // becomes the K-th key.  At the end of the item each list is made contiguous
 // (its second half moved up, or the whole list cut to K).
 constexpr int kQB = 2;  // 32-query blocks per wave
+constexpr uint32_t kShareTiles = 512;  // tiles between threshold exchanges without a compaction
 
 // Bit j (score j of the lane: row r0 + 8 (j / 4) + (j % 4), r0 = tile row +
 // 4 * half) set when that row lies in [qs, qe).
@@ -525,7 +526,8 @@ __device__ __attribute__((always_inline)) void scan_mfma_item(const ScanMfmaArgs
     const LayoutMfma &L = a.lay[W.layout];
     const uint4 *gA = reinterpret_cast<const uint4 *>(L.tiles);
     auto qi_of = [&](uint32_t j) { return a.iq[W.q_begin + q0 + j].qi; };
-    auto list_of = [&](uint32_t j) { return a.lists + (W.slot_begin + q0 + j) * (uint64_t)B; };
+    auto slot_of = [&](uint32_t j) { return list_slot(a.iq[W.q_begin + q0 + j], W); };
+    auto list_of = [&](uint32_t j) { return a.lists + slot_of(j) * (uint64_t)B; };
 
     // Start from the thresholds earlier items (or the prefix pass) published.
     for (uint32_t j = lane; j < nq; j += 64)
@@ -677,7 +679,7 @@ _Pragma("unroll")                                                               
 /* compacted with their other halves; every 64 tiles the thresholds    */                            \
 /* are shared with the other items of the same queries.                */                             \
         since_share += live ? 1u : 0u;                                                                \
-        if (appended || since_share >= 64) {                                                          \
+        if (appended || since_share >= kShareTiles) {                                                 \
             bool changed = false;                                                                     \
 _Pragma("unroll 1")                                                                                   \
             for (int b = 0; b < kQB; ++b) {                                                           \
@@ -700,13 +702,14 @@ _Pragma("unroll 1")                                                             
                     changed = true;                                                                   \
                 }                                                                                     \
             }                                                                                         \
-            if (since_share >= 64) {                                                                  \
+/* Share after a compaction (a threshold moved; the compaction has      */                           \
+/* already waited for this wave's memory traffic) and every kShareTiles  */                           \
+/* tiles (the fetched half: other items of the same queries may have     */                           \
+/* tightened theirs).  Each share waits for its atomics' results, and    */                           \
+/* with them for every row load in flight.                               */                           \
+            if (changed || since_share >= kShareTiles) {                                              \
                 since_share = 0;                                                                      \
                 share_thresholds<WaveSync>(a.gtau, nq, tau_w, qi_of);                                 \
-                changed = true;                                                                       \
-            }                                                                                         \
-            if (changed) {                                                                            \
-                WaveSync::sync();                                                                     \
 _Pragma("unroll")                                                                                     \
                 for (int b = 0; b < kQB; ++b)                                                         \
                     if (qv[b]) tauf[b] = tau_as_float(tau_w[32 * b + col]);                           \
@@ -745,7 +748,7 @@ _Pragma("unroll")                                                               
             WaveSync::sync();
             for (uint32_t i = lane; i < c1; i += 64) list[c0 + i] = list[H + i];
         }
-        if (lane == 0 && (a.qmask == nullptr || a.qmask[qi_of(j)] != 0)) a.counts[W.slot_begin + q0 + j] = count;
+        if (lane == 0 && (a.qmask == nullptr || a.qmask[qi_of(j)] != 0)) a.counts[slot_of(j)] = count;
     }
     share_thresholds<WaveSync>(a.gtau, nq, tau_w, qi_of);
 }

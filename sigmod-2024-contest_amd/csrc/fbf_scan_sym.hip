@@ -89,7 +89,7 @@ __device__ __attribute__((always_inline)) void scan_sym_item(const ScanSymArgs &
             const uint64_t key = ((uint64_t)(uint32_t)dist << 32) | id;
             uint64_t tau = s_tau[j];
             uint32_t cnt = s_cnt[j];
-            uint64_t *list = a.lists + (W.slot_begin + j) * (uint64_t)B;
+            uint64_t *list = a.lists + list_slot(e, W) * (uint64_t)B;
             offer_global<B>(list, cnt, tau, a.K, sbuf, in && key < tau, key);
             s_tau[j] = tau;
             s_cnt[j] = cnt;
@@ -98,9 +98,10 @@ __device__ __attribute__((always_inline)) void scan_sym_item(const ScanSymArgs &
     for (uint32_t j = 0; j < nq; ++j) {
         uint64_t tau = s_tau[j];
         uint32_t cnt = s_cnt[j];
-        uint64_t *list = a.lists + (W.slot_begin + j) * (uint64_t)B;
+        const uint64_t slot = list_slot(iq[j], W);
+        uint64_t *list = a.lists + slot * (uint64_t)B;
         compact_global<B>(list, cnt, tau, a.K, sbuf);
-        if (lane == 0) a.counts[W.slot_begin + j] = cnt;
+        if (lane == 0) a.counts[slot] = cnt;
     }
 }
 

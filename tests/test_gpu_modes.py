@@ -54,7 +54,7 @@ def test_mode_invariants_and_recall(pkg, orc, c0, mode):
     _say(f"case: config0 N=10000 nq=400 mode={mode} k=100 k'=400")
     vecs, labels, ts, q, metas, ds = c0
     lab = labels.astype(np.int32)
-    ids, d = _search(ds, q, metas, 100, mode)
+    ids, d = _search(ds, q, metas, 100, mode, kprime=400)
     eid, ed = orc.exact_predicate(vecs, lab, ts, q, metas, k=100, mode="canon")
     L = orc.Layouts(lab, ts)
     for i in range(q.shape[0]):
@@ -72,6 +72,25 @@ def test_mode_invariants_and_recall(pkg, orc, c0, mode):
         if cnt <= 400:
             assert np.array_equal(ids[i], eid[i]), (mode, i, cnt)
             assert np.array_equal(d[i], ed[i].astype(np.float32))
+    rec, _ = orc.recall(ids, eid)
+    _say(f"  recall@100 vs exact = {rec:.4f}")
+    assert rec >= RECALL_FLOOR[mode]
+
+
+@pytest.mark.parametrize("mode", ["bf16", "sq8"])
+def test_default_kprime(pkg, orc, c0, mode):
+    """Default over-fetch of the BF16 / SQ8 scans is 3k/2: queries admitting
+    at most 150 rows come back exact, and recall meets the floor."""
+    _say(f"case: config0 N=10000 nq=400 mode={mode} k=100 default k' (150)")
+    vecs, labels, ts, q, metas, ds = c0
+    lab = labels.astype(np.int32)
+    ids, d = _search(ds, q, metas, 100, mode)
+    eid, ed = orc.exact_predicate(vecs, lab, ts, q, metas, k=100, mode="canon")
+    L = orc.Layouts(lab, ts)
+    for i in range(q.shape[0]):
+        lay, s, e = L.query_range(*metas[i])
+        if e - s <= 150:
+            assert np.array_equal(ids[i], eid[i]), (mode, i)
     rec, _ = orc.recall(ids, eid)
     _say(f"  recall@100 vs exact = {rec:.4f}")
     assert rec >= RECALL_FLOOR[mode]
