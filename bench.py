@@ -49,6 +49,8 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--recall-sample", type=int, default=16, help="queries checked against FP64")
+    p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                   help="only run a plain device copy of MIB MiB (HBM counter calibration) and exit")
     return p.parse_args()
 
 
@@ -60,6 +62,18 @@ def main() -> int:
         return 2
     import torch
     import __graft_entry__ as ge
+
+    if a.calibrate_copy:
+        # A plain copy kernel of known size: under rocprofv3 --pmc FETCH_SIZE /
+        # WRITE_SIZE its counters give the bytes-per-counter-unit scale.
+        n = a.calibrate_copy * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": 4 * n, "launches": 3}), flush=True)
+        return 0
 
     pkg = ge.load_pkg()
     fbf, dg = pkg.fbf, pkg.datagen
@@ -147,14 +161,14 @@ def main() -> int:
         flops = 3.0 * 100 * pairs_per_launch
         achieved = flops / (avg_launch_ms * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS["fp32"], "unit": "TFLOP/s",
-                "frac": achieved / PEAK_TFLOPS["fp32"], "traffic": None}
+                "frac": achieved / PEAK_TFLOPS["fp32"], "traffic": committed_traffic(cfg.name, mode, a)}
     else:
         flops = 2.0 * 100 * pairs_per_launch
         pk = PEAK_TFLOPS["bf16"] if mode in ("bf16", "sq8") else PEAK_TFLOPS["fp8"] if mode == "fp8" else \
             PEAK_TFLOPS["int8"]
         achieved = flops / (avg_launch_ms * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": achieved, "peak": pk, "unit": "TFLOP/s",
-                "frac": achieved / pk, "traffic": None}
+                "frac": achieved / pk, "traffic": committed_traffic(cfg.name, mode, a)}
 
     cpu = None
     if not a.no_cpu:
@@ -184,6 +198,23 @@ def main() -> int:
     print(json.dumps(line), flush=True)
     ds.close()
     return 0
+
+
+def committed_traffic(workload, mode, a):
+    """HBM bytes per scan launch measured with rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, calibrated on a plain copy kernel) for the default run of this
+    workload and mode, from the committed profile summary; None otherwise."""
+    if a.kprime or a.k != 100 or a.chunk_rows or a.group_queries or a.prepass_div or a.prepass_margin or a.n or a.nq:
+        return None
+    name = f"scan_f32_exact_{workload}.md" if mode == "exact" else f"scan_mfma_{mode}_{workload}.md"
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
 
 
 def cpu_baseline(orc, vecs, labels, ts, q, metas, k, target_s):
