@@ -49,6 +49,8 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--recall-sample", type=int, default=16, help="queries checked against FP64")
+    p.add_argument("--autotune", action="store_true",
+                   help="pick group_queries / chunk_rows / kprime with the autotuner before timing")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                    help="only run a plain device copy of MIB MiB (HBM counter calibration) and exit")
     return p.parse_args()
@@ -99,6 +101,34 @@ def main() -> int:
            torch.empty((nq, a.k), dtype=torch.float32, device=dev))
     kw = dict(k=a.k, mode=mode, kprime=a.kprime, chunk_rows=a.chunk_rows, group_queries=a.group_queries,
               persistent=a.persistent, out=out, prepass_div=a.prepass_div, prepass_margin=a.prepass_margin)
+    tuned = None
+    if a.autotune and mode != "exact":
+        t0 = time.time()
+        res = pkg.autotune.tune(ds, qd, metas, k=a.k, mode=mode, log=log)
+        tuned = dict(res.opts, sample_recall=res.recall)
+        kw.update(res.opts)
+        log(f"autotune picked {res.opts} ({res.ms:.1f} ms, sample recall {res.recall:.4f}) in {time.time() - t0:.1f}s")
+
+    # Uneven batches (the C4 workload): the timed step runs all batches as one
+    # search -- one persistent launch per pass over a work queue of (query
+    # group, row chunk) items; for comparison the same batches are also run as
+    # one search each (untimed for the result line, logged).
+    per_batch = None
+    if cfg.uneven_batches:
+        sizes = dg.batch_sizes(cfg, nq)
+        off = np.concatenate([[0], np.cumsum(sizes)])
+        kwb = dict(kw)
+        kwb.pop("out")
+        ds.search(qd, metas, **kwb)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(len(sizes)):
+            ds.search(qd[off[i]:off[i + 1]], metas[off[i]:off[i + 1]], **kwb)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        per_batch = {"batches": len(sizes), "qps": nq / dt, "ms": 1e3 * dt}
+        log(f"{len(sizes)} uneven batches (sizes {min(sizes)}..{max(sizes)}) one search each: "
+            f"{nq / dt:.0f} QPS ({1e3 * dt:.1f} ms)")
 
     for i in range(a.warmup):
         ds.search(qd, metas, **kw)
@@ -188,7 +218,9 @@ def main() -> int:
         "dtype": {"exact": "f32", "sq8": "u8", "bf16": "bf16", "fp8": "fp8e4m3", "sq8sym": "i8"}[mode],
         "data": "synthetic (seeded Gaussian mixture, Zipf categories, uniform timestamps)",
         "config": {"workload": cfg.name, "n": n, "d": 100, "nq": nq, "k": a.k, "mode": mode,
-                   "kprime": a.kprime, "pairs_per_step": pairs, "work_items": stats["work_items"]},
+                   "kprime": kw["kprime"], "group_queries": kw["group_queries"], "chunk_rows": kw["chunk_rows"],
+                   "autotuned": tuned, "per_batch_searches": per_batch,
+                   "pairs_per_step": pairs, "work_items": stats["work_items"]},
         "recall_at_k_vs_fp64": rec,
         "recall_sample": int(sample.size),
         "max_rel_err_vs_fp64": kth_err,

@@ -7,6 +7,6 @@ The directory name is not a Python identifier; load it with ``load()`` from
 the repository root helpers (bench.py, tests/conftest.py), which registers it
 as ``fbf_amd``.
 """
-from . import datagen, fbf  # noqa: F401
+from . import autotune, datagen, fbf  # noqa: F401
 
-__all__ = ["fbf", "datagen"]
+__all__ = ["fbf", "datagen", "autotune"]

@@ -448,10 +448,13 @@ static fbf_status compute_ranges(const fbf_dataset *ds, const float *metas, uint
 }
 
 // Group queries and cut their spans into work items.  qr[i].qi must be i.
+constexpr uint64_t kTargetItems = 16384;  // automatic chunk size aims at this many work items
+
 // Work items from query ranges.  Queries of one layout are sorted by range
 // and packed greedily into groups of at most group_max whose shared span
 // exceeds the longest member's range by at most 25%; a group's span is cut
-// into chunks of chunk_rows, one work item each, and every item carries all
+// into chunks of chunk_rows (0: chosen from the total span), one work item
+// each, and every item carries all
 // members of its group (a member whose range misses the chunk costs one empty
 // list).  Query q owns nchunk[q] consecutive list slots, one per chunk of its
 // group; finish_slots numbers them.  Host work is O(nq log nq + items).
@@ -496,6 +499,11 @@ static void make_plan(const std::vector<QRange> &qr, uint32_t chunk_rows, uint32
         }
     };
 
+    struct Group {
+        size_t g0;
+        uint32_t gn, us, ue;
+    };
+    std::vector<Group> groups;
     size_t g0 = 0;  // current group: order[g0, i)
     uint32_t g_layout = 0, g_s = 0, g_e = 0, g_maxlen = 0;
     for (size_t i = 0; i < order.size(); ++i) {
@@ -508,7 +516,7 @@ static void make_plan(const std::vector<QRange> &qr, uint32_t chunk_rows, uint32
             fits = (uint64_t)(ne - ns) * 4 <= (uint64_t)ml * 5 + 4 * kRowTile;
         }
         if (!fits) {
-            if (i > g0) emit_group(order.data() + g0, (uint32_t)(i - g0), g_s, g_e);
+            if (i > g0) groups.push_back(Group{g0, (uint32_t)(i - g0), g_s, g_e});
             g0 = i;
             g_layout = r.layout;
             g_s = r.s;
@@ -519,7 +527,17 @@ static void make_plan(const std::vector<QRange> &qr, uint32_t chunk_rows, uint32
         g_e = std::max(g_e, r.e);
         g_maxlen = std::max(g_maxlen, r.e - r.s);
     }
-    if (order.size() > g0) emit_group(order.data() + g0, (uint32_t)(order.size() - g0), g_s, g_e);
+    if (order.size() > g0) groups.push_back(Group{g0, (uint32_t)(order.size() - g0), g_s, g_e});
+    if (chunk_rows == 0) {
+        // Automatic chunk: about kTargetItems items over all groups (enough
+        // independent work to fill the GPU for small batches too), 16 Ki to
+        // 256 Ki rows.
+        uint64_t span = 0;
+        for (const Group &g : groups) span += g.ue - g.us / kRowTile * kRowTile;
+        const uint64_t c = (span / kTargetItems + kRowTile - 1) / kRowTile * kRowTile;
+        chunk_rows = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(c, 16384), 262144);
+    }
+    for (const Group &g : groups) emit_group(order.data() + g.g0, g.gn, g.us, g.ue);
 }
 
 // Number the list slots of a one-pass plan: query q owns nchunk[q] slots from
@@ -685,9 +703,8 @@ extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, cons
         if (K < k) return fail(FBF_ERR_INVALID_ARG, "kprime must be >= k");
         if (K > 896) return fail(FBF_ERR_INVALID_ARG, "kprime must be <= 896");
     }
-    uint32_t chunk = o.chunk_rows ? (o.chunk_rows + kRowTile - 1) / kRowTile * kRowTile : 256u * 1024u;
-    // The MFMA scans stage candidates with a 26-bit row offset in the item.
-    if (mfma) chunk = std::min<uint32_t>(chunk, 1u << 24);
+    // Rows per work item (0: the planner picks it from the batch).
+    const uint32_t chunk = o.chunk_rows ? (o.chunk_rows + kRowTile - 1) / kRowTile * kRowTile : 0u;
     // MFMA items carry up to 256 queries (4 waves x 64), 128 by default (two
     // waves share each row tile); VALU items up to 16.
     const uint32_t gcap = mfma ? 256u : (uint32_t)kMaxGroup;

@@ -38,6 +38,7 @@ class Config:
     width_hi: float
     width_log: bool = False
     seed: int = 0
+    uneven_batches: bool = False  # queries arrive in batches of log-uniform size (batch_sizes)
 
 
 CONFIGS = {
@@ -46,7 +47,7 @@ CONFIGS = {
     # Full scale: N=10M, 100k queries (exact FP32 / SQ8 / BF16 / FP8 share it).
     1: Config("c1_n10m_q100k", 10_000_000, 100_000, 1024, 1000, 1.1, 0.01, 0.5, False, 1),
     # Uneven persistent batch: 10k Zipf(1.3) categories, log-uniform widths.
-    4: Config("c4_n10m_zipf13", 10_000_000, 100_000, 1024, 10_000, 1.3, 0.001, 0.8, True, 4),
+    4: Config("c4_n10m_zipf13", 10_000_000, 100_000, 1024, 10_000, 1.3, 0.001, 0.8, True, 4, True),
 }
 
 
