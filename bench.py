@@ -49,8 +49,9 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--recall-sample", type=int, default=16, help="queries checked against FP64")
-    p.add_argument("--autotune", action="store_true",
-                   help="pick group_queries / chunk_rows / kprime with the autotuner before timing")
+    p.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True,
+                   help="pick group_queries / chunk_rows / kprime with the project's autotuner (untimed) "
+                        "unless they are given (approximate modes)")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                    help="only run a plain device copy of MIB MiB (HBM counter calibration) and exit")
     return p.parse_args()
@@ -102,7 +103,7 @@ def main() -> int:
     kw = dict(k=a.k, mode=mode, kprime=a.kprime, chunk_rows=a.chunk_rows, group_queries=a.group_queries,
               persistent=a.persistent, out=out, prepass_div=a.prepass_div, prepass_margin=a.prepass_margin)
     tuned = None
-    if a.autotune and mode != "exact":
+    if a.autotune and mode != "exact" and not (a.kprime or a.chunk_rows or a.group_queries):
         t0 = time.time()
         res = pkg.autotune.tune(ds, qd, metas, k=a.k, mode=mode, log=log)
         tuned = dict(res.opts, sample_recall=res.recall)
@@ -191,14 +192,14 @@ def main() -> int:
         flops = 3.0 * 100 * pairs_per_launch
         achieved = flops / (avg_launch_ms * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS["fp32"], "unit": "TFLOP/s",
-                "frac": achieved / PEAK_TFLOPS["fp32"], "traffic": committed_traffic(cfg.name, mode, a)}
+                "frac": achieved / PEAK_TFLOPS["fp32"], "traffic": committed_traffic(cfg.name, mode, a, kw)}
     else:
         flops = 2.0 * 100 * pairs_per_launch
         pk = PEAK_TFLOPS["bf16"] if mode in ("bf16", "sq8") else PEAK_TFLOPS["fp8"] if mode == "fp8" else \
             PEAK_TFLOPS["int8"]
         achieved = flops / (avg_launch_ms * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": achieved, "peak": pk, "unit": "TFLOP/s",
-                "frac": achieved / pk, "traffic": committed_traffic(cfg.name, mode, a)}
+                "frac": achieved / pk, "traffic": committed_traffic(cfg.name, mode, a, kw)}
 
     cpu = None
     if not a.no_cpu:
@@ -232,21 +233,25 @@ def main() -> int:
     return 0
 
 
-def committed_traffic(workload, mode, a):
+def committed_traffic(workload, mode, a, kw):
     """HBM bytes per scan launch measured with rocprofv3 --pmc (FETCH_SIZE +
-    WRITE_SIZE, calibrated on a plain copy kernel) for the default run of this
-    workload and mode, from the committed profile summary; None otherwise."""
-    if a.kprime or a.k != 100 or a.chunk_rows or a.group_queries or a.prepass_div or a.prepass_margin or a.n or a.nq:
+    WRITE_SIZE, calibrated on a plain copy kernel) for this workload, mode and
+    search options, from the committed profile summary; None otherwise."""
+    if a.k != 100 or a.prepass_div or a.prepass_margin or a.n or a.nq:
         return None
     name = f"scan_f32_exact_{workload}.md" if mode == "exact" else f"scan_mfma_{mode}_{workload}.md"
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
+    opts = {key: int(kw[key]) for key in ("group_queries", "chunk_rows", "kprime")}
+    traffic, profiled = None, None
     with open(path) as f:
         for line in f:
             if line.startswith("traffic_bytes_per_launch:"):
-                return float(line.split(":", 1)[1])
-    return None
+                traffic = float(line.split(":", 1)[1])
+            if line.startswith("search_options:"):
+                profiled = json.loads(line.split(":", 1)[1])
+    return traffic if profiled == opts else None
 
 
 def cpu_baseline(orc, vecs, labels, ts, q, metas, k, target_s):

@@ -13,7 +13,10 @@ representation.  Each candidate is timed on the whole batch (one warm-up
 search, then the mean of `repeats` searches), since the item plan and the
 threshold seeding depend on the whole batch.  Candidates are tried in order
 of increasing k' within each tile shape, and a tile shape stops at the first
-k' that meets the target: larger k' only costs time.
+k' that meets the target: larger k' only costs time.  Among the candidates
+that meet the target, the first one tried (smaller tile width, then fewer
+rows per item) whose time is within `tie` of the fastest wins, so that timing
+noise does not flip the choice between near-equal shapes.
 """
 from __future__ import annotations
 
@@ -60,7 +63,8 @@ def recall_at_k(ids: np.ndarray, ref: np.ndarray) -> float:
 
 def tune(ds, queries, metas, k: int = 100, mode: str = "bf16", recall_target: float = 0.995,
          sample: int = 256, groups=(64, 128, 256), chunks=(131072, 262144, 524288),
-         kprime_factors=(1.0, 1.2, 1.5, 2.0, 4.0), repeats: int = 1, log=None) -> TuneResult:
+         kprime_factors=(1.0, 1.2, 1.5, 2.0, 4.0), repeats: int = 1, tie: float = 0.03,
+         log=None) -> TuneResult:
     """queries: torch device float32 [nq, d]; metas: numpy float32 [nq, 4]."""
     import torch
     metas = np.ascontiguousarray(metas, np.float32)
@@ -82,7 +86,7 @@ def tune(ds, queries, metas, k: int = 100, mode: str = "bf16", recall_target: fl
         return ms, recall_at_k(got, ref)
 
     table = []
-    best = None
+    ok = []
     for g, c in itertools.product(groups, chunks):
         for f in kprime_factors:
             kp = min(896, max(k, int(round(f * k))))
@@ -92,9 +96,11 @@ def tune(ds, queries, metas, k: int = 100, mode: str = "bf16", recall_target: fl
             if log:
                 log(f"autotune {mode}: {opts} {ms:.1f} ms recall {rec:.4f}")
             if rec >= recall_target:
-                if best is None or ms < best[1]:
-                    best = (opts, ms, rec)
+                ok.append((opts, ms, rec))
                 break
-    if best is None:  # nothing met the target: the most accurate candidate
+    if ok:
+        fastest = min(r[1] for r in ok)
+        best = next(r for r in ok if r[1] <= fastest * (1.0 + tie))
+    else:  # nothing met the target: the most accurate candidate
         best = max(table, key=lambda r: (r[2], -r[1]))
     return TuneResult(opts=best[0], ms=best[1], recall=best[2], table=table)

@@ -448,7 +448,7 @@ static fbf_status compute_ranges(const fbf_dataset *ds, const float *metas, uint
 }
 
 // Group queries and cut their spans into work items.  qr[i].qi must be i.
-constexpr uint64_t kTargetItems = 16384;  // automatic chunk size aims at this many work items
+constexpr uint64_t kTargetItems = 32768;  // automatic chunk size aims at this many work items
 
 // Work items from query ranges.  Queries of one layout are sorted by range
 // and packed greedily into groups of at most group_max whose shared span

