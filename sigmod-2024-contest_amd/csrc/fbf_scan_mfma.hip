@@ -633,13 +633,17 @@ _Pragma("unroll")                                                               
                 float sc[16];                                                                         \
 _Pragma("unroll")                                                                                     \
                 for (int j = 0; j < 16; ++j) sc[j] = VAR == 1 ? acc[j] + xn[j] : acc[j];              \
-/* One compare per lane: the smallest of its 16 scores. */                                            \
+/* One compare per lane: the smallest of its 16 scores against the    */                           \
+/* lane's threshold, or against -inf when the tile lies wholly outside */                            \
+/* the lane's query range (the block runs for the other queries).      */                            \
                 float mn = sc[0];                                                                     \
 _Pragma("unroll")                                                                                     \
                 for (int j = 1; j < 16; ++j) mn = fminf(mn, sc[j]);                                   \
-                if (__ballot(mn <= tauf[b]) == 0) continue;                                           \
+                const bool full = t >= qs[b] && t + kMTile <= qe[b];                                  \
+                const float th = (t < qe[b] && t + kMTile > qs[b]) ? tauf[b] : -__builtin_inff();     \
+                if (__ballot(mn <= th) == 0) continue;                                                \
 /* Rare path: a per-lane bit mask of the scores at or below the lane's  */                           \
-/* threshold (inside its query's range on a partial tile); each lane     */                           \
+/* threshold (inside its query's range on a boundary tile); each lane    */                           \
 /* then stores its set bits to its half of the list, lowest first.  A    */                           \
 /* score equal to the threshold may be stored needlessly; compaction     */                           \
 /* drops it.  Keys carry the layout position (bit 31: layout), not the   */                           \
@@ -647,8 +651,8 @@ _Pragma("unroll")                                                               
                 appended = true;                                                                      \
                 uint32_t pm = 0;                                                                      \
 _Pragma("unroll")                                                                                     \
-                for (int j = 0; j < 16; ++j) pm |= (sc[j] <= tauf[b] ? 1u : 0u) << j;                 \
-                if (__ballot(qv[b] && !(t >= qs[b] && t + kMTile <= qe[b]))) pm &= range_bits(t, half, qs[b], qe[b]); \
+                for (int j = 0; j < 16; ++j) pm |= (sc[j] <= th ? 1u : 0u) << j;                      \
+                if (__ballot(pm != 0 && !full)) pm &= range_bits(t, half, qs[b], qe[b]);              \
                 append_bits(pm, sc, t + 4 * half, lay_bit, mylist[b], cnt[b]);                        \
             }                                                                                         \
         }                                                                                             \
