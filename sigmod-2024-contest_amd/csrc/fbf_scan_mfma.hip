@@ -416,6 +416,49 @@ __device__ inline f32x16 tile_mma(const RowFrag<VAR> &r, const QFrag<VAR> &q) {
     return acc;
 }
 
+// Row tiles reach LDS by direct loads (global_load_lds_dwordx4): each lane
+// fetches 16 bytes and the wave writes them as one 1 KiB chunk at a
+// wave-uniform LDS address, with no VGPR staging.  A tile is kChunks chunks;
+// an LDS tile buffer is padded to whole chunks (the SQ8 tile is 3.5 chunks:
+// the pad takes copies of the tile's last piece and is never read).
+template <int VAR> struct TileChunks {
+    static constexpr int value = (TileBytes<VAR>::value / 16 + 63) / 64;
+};
+
+// Issue this wave's share of one tile's chunks: chunk w, w + NW, ...; every
+// wave issues the same number of loads (indices clamped: a clamped chunk is
+// written twice with the same bytes), so each step's wait count is a constant.
+template <int VAR, int NW>
+__device__ __attribute__((always_inline)) inline void dma_tile(const uint4 *src, uint4 *dst, uint32_t wave,
+                                                               uint32_t lane) {
+    constexpr int kU4 = TileBytes<VAR>::value / 16;
+    constexpr int kC = TileChunks<VAR>::value;
+    constexpr int kPerW = (kC + NW - 1) / NW;
+#pragma unroll
+    for (int i = 0; i < kPerW; ++i) {
+        const uint32_t c = min(wave + (uint32_t)(i * NW), (uint32_t)kC - 1);
+        const uint32_t e = min(c * 64 + lane, (uint32_t)kU4 - 1);
+        __builtin_amdgcn_global_load_lds(src + e, (__attribute__((address_space(3))) void *)(dst + c * 64), 16, 0, 0);
+    }
+}
+
+// s_waitcnt vmcnt(N) alone (gfx9 encoding: vmcnt in bits 3:0 and 15:14,
+// expcnt and lgkmcnt at their maxima).
+template <int N> __device__ __attribute__((always_inline)) inline void wait_vm() {
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
+}
+
+// Workgroup barrier for the tile ring.  The direct loads count in vmcnt and
+// each wave has waited for its own share of the next tile before arriving, so
+// after the barrier every wave may read the whole tile.  No memory fence: a
+// fence would also wait for the tiles still in flight.
+__device__ __attribute__((always_inline)) inline void ring_barrier() {
+    __asm__ volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    __asm__ volatile("" ::: "memory");
+}
+
 __device__ inline float tau_as_float(uint64_t tau) {
     return tau == kKeyMax ? __builtin_inff() : ord2f((uint32_t)(tau >> 32));
 }
@@ -512,10 +555,11 @@ __device__ inline void compact_split(uint64_t *list, uint32_t &c0, uint32_t &c1,
 }
 
 template <int VAR, int NW, int B>
-__device__ __attribute__((always_inline)) void scan_mfma_item(const ScanMfmaArgs &a, const WorkItem &W, uint4 (*sA)[TileBytes<VAR>::value / 16],
-                               uint64_t *sbuf, uint64_t *s_tau) {
+__device__ __attribute__((always_inline)) void scan_mfma_item(const ScanMfmaArgs &a, const WorkItem &W, uint4 *sT0,
+                                                               uint4 *sT1, uint4 *sT2, uint64_t *sbuf,
+                                                               uint64_t *s_tau) {
     constexpr int kU4 = TileBytes<VAR>::value / 16;
-    constexpr int kPer = (kU4 + 64 * NW - 1) / (64 * NW);  // staging loads per thread
+    constexpr int kPerW = (TileChunks<VAR>::value + NW - 1) / NW;  // direct loads per wave and tile
     constexpr uint32_t H = B / 2;                          // keys per list half
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t col = lane & 31, half = lane >> 5;
@@ -559,50 +603,29 @@ __device__ __attribute__((always_inline)) void scan_mfma_item(const ScanMfmaArgs
         mylist[b] = list_of(j) + half * H;
     }
 
-    // Rows stream through LDS.  kDepth tiles are in flight in registers (each
-    // thread holds kPer 16-byte pieces of every tile), one tile sits in LDS for
-    // the matrix cores, and a tile's loads are issued kDepth tiles before its
-    // math so their latency hides behind the work in between.  The ring is
-    // indexed with compile-time slots (the loop body is unrolled kDepth times).
-    constexpr int kDepth = kPer <= 1 ? 6 : kPer <= 2 ? 4 : kPer <= 4 ? 2 : 1;
+    // Rows stream through three LDS tile buffers: while the matrix cores work
+    // on tile m, the direct loads of tiles m + 1 and m + 2 are in flight.
+    // The buffers are separate LDS variables and the loop body is unrolled
+    // three times, so each step names its buffers at compile time and the
+    // compiler can tell the buffer being read from the ones being filled
+    // (it adds no wait for the loads in flight before the LDS reads).
     const uint32_t ntile = (W.row_end - W.row_begin + kMTile - 1) / kMTile;
-    // Plain vector type for the ring: the HIP uint4 wrapper struct keeps the
-    // compiler from promoting the array to registers.
-    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-    const u32x4 *gbase = reinterpret_cast<const u32x4 *>(gA) + (uint64_t)(W.row_begin / kMTile) * kU4;
-    u32x4 (*sAv)[kU4] = reinterpret_cast<u32x4 (*)[kU4]>(sA);
-    u32x4 ring[kDepth][kPer];
-    // Tile 0 straight to LDS, tiles 1..kDepth into the ring (every item has
-    // at least one tile: the planner emits no empty items).  The ring loads
-    // are unconditional, as in the loop below (indices clamped).
-#pragma unroll
-    for (int i = 0; i < kPer; ++i) {
-        const uint32_t e = tid + i * 64 * NW;
-        if (e < (uint32_t)kU4) sAv[0][e] = gbase[e];
-    }
-#pragma unroll
-    for (int d = 0; d < kDepth; ++d) {
-        const uint32_t m = min(1u + d, ntile - 1);
-#pragma unroll
-        for (int i = 0; i < kPer; ++i) {
-            const uint32_t e = min(tid + i * 64 * NW, (uint32_t)kU4 - 1);
-            ring[(1 + d) % kDepth][i] = gbase[(uint64_t)m * kU4 + e];
-        }
-        // Issue order = tile order, as in the loop, so the waits computed for
-        // the loop's first step match those of its later steps.
-        __builtin_amdgcn_sched_barrier(0);
-    }
-    __syncthreads();
+    const uint4 *gbase = gA + (uint64_t)(W.row_begin / kMTile) * kU4;
+#define FBF_BUF(i) ((i) == 0 ? sT0 : (i) == 1 ? sT1 : sT2)
+    // Every item has at least one tile (the planner emits no empty items).
+    dma_tile<VAR, NW>(gbase, sT0, wave, lane);
+    dma_tile<VAR, NW>(gbase + (uint64_t)min(1u, ntile - 1) * kU4, sT1, wave, lane);
+    wait_vm<kPerW>();  // tile 0
+    ring_barrier();
 
     const uint32_t lay_bit = W.layout << 31;
     uint32_t since_share = 0;
-    // The loop body, once per ring slot with a literal slot number so the
-    // ring stays in registers (a runtime index would put it in scratch).
+    // The loop body, once per tile buffer with a literal buffer number.
 #define FBF_TILE_STEP(U)                                                                              \
     {                                                                                                 \
         const uint32_t m = m0 + (U);                                                                  \
-/* Steps past the last tile (the last pass of the loop runs all kDepth  */                            \
-/* steps) compute nothing but keep staging, so that every pass issues    */                           \
+/* Steps past the last tile (the last pass of the loop runs all three   */                            \
+/* steps) compute nothing but keep loading, so that every pass issues    */                           \
 /* the same loads in the same order and the loop has no exit in between. */                           \
         const bool live = m < ntile;                                                                  \
         const uint32_t t = W.row_begin + m * kMTile;                                                  \
@@ -613,19 +636,29 @@ _Pragma("unroll")                                                               
             hit[b] = __ballot(live && qv[b] && t < qe[b] && t + kMTile > qs[b]);                      \
             hit_any |= hit[b] != 0;                                                                   \
         }                                                                                             \
-        bool appended = false;                                                                        \
-        if (hit_any) {                                                                                \
-            RowFrag<VAR> rf;                                                                          \
-            frag_from_lds<VAR>(sA[m & 1u], lane, rf);                                                 \
-            float xn[16];                                                                             \
-            if constexpr (VAR == 1) {                                                                 \
-                rf.scale = (int)L.scale[t + col] * 0x01010101;                                        \
+/* FP8 row scales and norms: loaded before the tile's direct loads are */                            \
+/* issued, so waiting for them does not wait for the tile.              */                            \
+        float xn[16];                                                                                 \
+        int rscale = 0;                                                                               \
+        if constexpr (VAR == 1) {                                                                     \
+            if (hit_any) {                                                                            \
+                rscale = (int)L.scale[t + col] * 0x01010101;                                          \
 _Pragma("unroll")                                                                                     \
                 for (int g = 0; g < 4; ++g) {                                                         \
                     const float4 v = *reinterpret_cast<const float4 *>(L.xnorm + t + 8 * g + 4 * half); \
                     xn[4 * g + 0] = v.x; xn[4 * g + 1] = v.y; xn[4 * g + 2] = v.z; xn[4 * g + 3] = v.w; \
                 }                                                                                     \
             }                                                                                         \
+        }                                                                                             \
+/* Tile m + 2 into the buffer tile m - 1 left (every wave is past the  */                            \
+/* barrier that ended step m - 1).  Steps past the last tile re-load    */                            \
+/* the last tile, so every step issues the same loads.                  */                            \
+        dma_tile<VAR, NW>(gbase + (uint64_t)min(m + 2, ntile - 1) * kU4, FBF_BUF(((U) + 2) % 3), wave, lane); \
+        bool appended = false;                                                                        \
+        if (hit_any) {                                                                                \
+            RowFrag<VAR> rf;                                                                          \
+            frag_from_lds<VAR>(FBF_BUF(U), lane, rf);                                                 \
+            if constexpr (VAR == 1) rf.scale = rscale;                                                \
 _Pragma("unroll")                                                                                     \
             for (int b = 0; b < kQB; ++b) {                                                           \
                 if (hit[b] == 0) continue;                                                            \
@@ -654,28 +687,6 @@ _Pragma("unroll")                                                               
                 for (int j = 0; j < 16; ++j) pm |= (sc[j] <= th ? 1u : 0u) << j;                      \
                 if (__ballot(pm != 0 && !full)) pm &= range_bits(t, half, qs[b], qe[b]);              \
                 append_bits(pm, sc, t + 4 * half, lay_bit, mylist[b], cnt[b]);                        \
-            }                                                                                         \
-        }                                                                                             \
-/* Tile m + 1 (issued kDepth tiles ago) goes to LDS; its ring slot */                                 \
-/* takes the loads of tile m + 1 + kDepth. */                                                         \
-        {                                                                                             \
-            const int slot = ((U) + 1) % kDepth;                                                      \
-            if (m + 1 < ntile) {                                                                      \
-_Pragma("unroll")                                                                                     \
-                for (int i = 0; i < kPer; ++i) {                                                      \
-                    const uint32_t e = tid + i * 64 * NW;                                             \
-                    if (e < (uint32_t)kU4) sAv[(m + 1) & 1u][e] = ring[slot][i];                      \
-                }                                                                                     \
-            }                                                                                         \
-/* Every thread issues the same number of loads on every step (tile and */                            \
-/* piece indices clamped: the extra loads re-read valid bytes and are    */                           \
-/* never used), so the compiler's wait before the LDS store above counts */                           \
-/* only this slot's loads and leaves the later tiles' loads in flight.   */                           \
-            const uint32_t mn2 = min(m + 1 + kDepth, ntile - 1);                                      \
-_Pragma("unroll")                                                                                     \
-            for (int i = 0; i < kPer; ++i) {                                                          \
-                const uint32_t e = min(tid + i * 64 * NW, (uint32_t)kU4 - 1);                         \
-                ring[slot][i] = gbase[(uint64_t)mn2 * kU4 + e];                                       \
             }                                                                                         \
         }                                                                                             \
 /* Wave-local list maintenance (no cross-wave synchronisation): halves */                             \
@@ -719,18 +730,19 @@ _Pragma("unroll")                                                               
                     if (qv[b]) tauf[b] = tau_as_float(tau_w[32 * b + col]);                           \
             }                                                                                         \
         }                                                                                             \
-        __syncthreads(); /* tile m + 1 staged; tile m's buffer free for tile m + 2 */                 \
+/* This wave's loads of tile m + 1 (issued one step ago) have landed;  */                           \
+/* the barrier makes the whole tile visible and frees tile m's buffer.  */                           \
+        wait_vm<kPerW>();                                                                             \
+        ring_barrier();                                                                               \
     }
 #pragma unroll 1
-    for (uint32_t m0 = 0; m0 < ntile; m0 += kDepth) {
+    for (uint32_t m0 = 0; m0 < ntile; m0 += 3) {
         FBF_TILE_STEP(0)
-        if constexpr (kDepth > 1) FBF_TILE_STEP(1)
-        if constexpr (kDepth > 2) FBF_TILE_STEP(2)
-        if constexpr (kDepth > 3) FBF_TILE_STEP(3)
-        if constexpr (kDepth > 4) FBF_TILE_STEP(4)
-        if constexpr (kDepth > 5) FBF_TILE_STEP(5)
+        FBF_TILE_STEP(1)
+        FBF_TILE_STEP(2)
     }
 #undef FBF_TILE_STEP
+#undef FBF_BUF
     // Final cut: each list becomes contiguous, [0, count): longer than K ->
     // sorted and cut to K; otherwise its second half moves up behind the
     // first (unsorted: the merge does not need order).
@@ -776,7 +788,8 @@ __device__ inline uint32_t xcc_id() {
 template <int VAR, int NW, int B>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2, 8)))
 void scan_mfma_kernel(ScanMfmaArgs a) {
-    __shared__ uint4 sA[2][TileBytes<VAR>::value / 16];
+    __shared__ uint4 sT0[64 * TileChunks<VAR>::value], sT1[64 * TileChunks<VAR>::value],
+        sT2[64 * TileChunks<VAR>::value];
     __shared__ uint64_t sbuf[NW][B];
     __shared__ uint64_t s_tau[64 * NW];
     __shared__ uint32_t s_idx;
@@ -814,7 +827,7 @@ void scan_mfma_kernel(ScanMfmaArgs a) {
         __syncthreads();
         if (idx == kPadId) break;
         const WorkItem W = as_const(a.items)[idx];
-        if (item_active<NW>(a, W)) scan_mfma_item<VAR, NW, B>(a, W, sA, my_sbuf, s_tau);
+        if (item_active<NW>(a, W)) scan_mfma_item<VAR, NW, B>(a, W, sT0, sT1, sT2, my_sbuf, s_tau);
         __syncthreads();
     }
 }
