@@ -33,7 +33,7 @@ def log(msg: str) -> None:
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--config", type=int, default=1)
     p.add_argument("--mode", default="bf16", choices=["exact", "sq8", "bf16", "fp8", "sq8sym"])

@@ -464,10 +464,10 @@ __device__ inline float tau_as_float(uint64_t tau) {
 }
 
 // One workgroup of NW waves per work item.  The item's rows stream through a
-// double-buffered LDS tile (loaded once, read by every wave); wave w owns the
-// item's queries [64 w, 64 w + 64) as two 32-query B blocks in VGPRs, its own
+// ring of LDS tiles (loaded once, read by every wave); wave w owns the item's
+// queries [64 w, 64 w + 64) as two 32-query B blocks in VGPRs, its own
 // candidate lists and thresholds, and synchronises with the other waves only
-// at the one barrier per tile that hands the next tile over.
+// at the one barrier per pair of tiles that hands the next pair over.
 //
 // Candidate lists.  In the MFMA output tile a query's 32 rows sit in two lanes
 // (l and l + 32, 16 rows each), so each query's list of B keys is split in two
@@ -556,10 +556,9 @@ __device__ inline void compact_split(uint64_t *list, uint32_t &c0, uint32_t &c1,
 
 template <int VAR, int NW, int B>
 __device__ __attribute__((always_inline)) void scan_mfma_item(const ScanMfmaArgs &a, const WorkItem &W, uint4 *sT0,
-                                                               uint4 *sT1, uint4 *sT2, uint64_t *sbuf,
+                                                               uint4 *sT1, uint4 *sT2, uint4 *sT3, uint64_t *sbuf,
                                                                uint64_t *s_tau) {
     constexpr int kU4 = TileBytes<VAR>::value / 16;
-    constexpr int kPerW = (TileChunks<VAR>::value + NW - 1) / NW;  // direct loads per wave and tile
     constexpr uint32_t H = B / 2;                          // keys per list half
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t col = lane & 31, half = lane >> 5;
@@ -603,19 +602,21 @@ __device__ __attribute__((always_inline)) void scan_mfma_item(const ScanMfmaArgs
         mylist[b] = list_of(j) + half * H;
     }
 
-    // Rows stream through three LDS tile buffers: while the matrix cores work
-    // on tile m, the direct loads of tiles m + 1 and m + 2 are in flight.
-    // The buffers are separate LDS variables and the loop body is unrolled
-    // three times, so each step names its buffers at compile time and the
-    // compiler can tell the buffer being read from the ones being filled
-    // (it adds no wait for the loads in flight before the LDS reads).
+    // Rows stream through four LDS tile buffers in pairs of tiles: while the
+    // matrix cores work on a pair, the direct loads of the next pair are in
+    // flight, and the waves synchronise once per pair (a slow step of one
+    // wave is absorbed over two tiles).  The buffers are separate LDS
+    // variables and the loop body is unrolled four times, so each step names
+    // its buffers at compile time and the compiler can tell the buffers being
+    // read from the ones being filled (it adds no wait for the loads in flight
+    // before the LDS reads).
     const uint32_t ntile = (W.row_end - W.row_begin + kMTile - 1) / kMTile;
     const uint4 *gbase = gA + (uint64_t)(W.row_begin / kMTile) * kU4;
-#define FBF_BUF(i) ((i) == 0 ? sT0 : (i) == 1 ? sT1 : sT2)
+#define FBF_BUF(i) ((i) == 0 ? sT0 : (i) == 1 ? sT1 : (i) == 2 ? sT2 : sT3)
     // Every item has at least one tile (the planner emits no empty items).
     dma_tile<VAR, NW>(gbase, sT0, wave, lane);
     dma_tile<VAR, NW>(gbase + (uint64_t)min(1u, ntile - 1) * kU4, sT1, wave, lane);
-    wait_vm<kPerW>();  // tile 0
+    wait_vm<0>();  // tiles 0 and 1
     ring_barrier();
 
     const uint32_t lay_bit = W.layout << 31;
@@ -624,7 +625,7 @@ __device__ __attribute__((always_inline)) void scan_mfma_item(const ScanMfmaArgs
 #define FBF_TILE_STEP(U)                                                                              \
     {                                                                                                 \
         const uint32_t m = m0 + (U);                                                                  \
-/* Steps past the last tile (the last pass of the loop runs all three   */                            \
+/* Steps past the last tile (the last pass of the loop runs all four    */                            \
 /* steps) compute nothing but keep loading, so that every pass issues    */                           \
 /* the same loads in the same order and the loop has no exit in between. */                           \
         const bool live = m < ntile;                                                                  \
@@ -650,10 +651,14 @@ _Pragma("unroll")                                                               
                 }                                                                                     \
             }                                                                                         \
         }                                                                                             \
-/* Tile m + 2 into the buffer tile m - 1 left (every wave is past the  */                            \
-/* barrier that ended step m - 1).  Steps past the last tile re-load    */                            \
-/* the last tile, so every step issues the same loads.                  */                            \
-        dma_tile<VAR, NW>(gbase + (uint64_t)min(m + 2, ntile - 1) * kU4, FBF_BUF(((U) + 2) % 3), wave, lane); \
+/* First step of a pair: tiles m + 2 and m + 3 into the buffers the   */                            \
+/* previous pair left (every wave is past the barrier that ended it).   */                            \
+/* Steps past the last tile re-load the last tile, so every pair issues */                            \
+/* the same loads.                                                      */                            \
+        if constexpr ((U) % 2 == 0) {                                                                 \
+            dma_tile<VAR, NW>(gbase + (uint64_t)min(m + 2, ntile - 1) * kU4, FBF_BUF(((U) + 2) % 4), wave, lane); \
+            dma_tile<VAR, NW>(gbase + (uint64_t)min(m + 3, ntile - 1) * kU4, FBF_BUF(((U) + 3) % 4), wave, lane); \
+        }                                                                                             \
         bool appended = false;                                                                        \
         if (hit_any) {                                                                                \
             RowFrag<VAR> rf;                                                                          \
@@ -730,16 +735,20 @@ _Pragma("unroll")                                                               
                     if (qv[b]) tauf[b] = tau_as_float(tau_w[32 * b + col]);                           \
             }                                                                                         \
         }                                                                                             \
-/* This wave's loads of tile m + 1 (issued one step ago) have landed;  */                           \
-/* the barrier makes the whole tile visible and frees tile m's buffer.  */                           \
-        wait_vm<kPerW>();                                                                             \
-        ring_barrier();                                                                               \
+/* Second step of a pair: this wave's loads of the next pair (issued at */                           \
+/* the start of this pair) have landed; the barrier makes those tiles    */                           \
+/* visible to every wave and frees this pair's buffers.                  */                           \
+        if constexpr ((U) % 2 == 1) {                                                                 \
+            wait_vm<0>();                                                                             \
+            ring_barrier();                                                                           \
+        }                                                                                             \
     }
 #pragma unroll 1
-    for (uint32_t m0 = 0; m0 < ntile; m0 += 3) {
+    for (uint32_t m0 = 0; m0 < ntile; m0 += 4) {
         FBF_TILE_STEP(0)
         FBF_TILE_STEP(1)
         FBF_TILE_STEP(2)
+        FBF_TILE_STEP(3)
     }
 #undef FBF_TILE_STEP
 #undef FBF_BUF
@@ -789,7 +798,7 @@ template <int VAR, int NW, int B>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2, 8)))
 void scan_mfma_kernel(ScanMfmaArgs a) {
     __shared__ uint4 sT0[64 * TileChunks<VAR>::value], sT1[64 * TileChunks<VAR>::value],
-        sT2[64 * TileChunks<VAR>::value];
+        sT2[64 * TileChunks<VAR>::value], sT3[64 * TileChunks<VAR>::value];
     __shared__ uint64_t sbuf[NW][B];
     __shared__ uint64_t s_tau[64 * NW];
     __shared__ uint32_t s_idx;
@@ -827,7 +836,7 @@ void scan_mfma_kernel(ScanMfmaArgs a) {
         __syncthreads();
         if (idx == kPadId) break;
         const WorkItem W = as_const(a.items)[idx];
-        if (item_active<NW>(a, W)) scan_mfma_item<VAR, NW, B>(a, W, sT0, sT1, sT2, my_sbuf, s_tau);
+        if (item_active<NW>(a, W)) scan_mfma_item<VAR, NW, B>(a, W, sT0, sT1, sT2, sT3, my_sbuf, s_tau);
         __syncthreads();
     }
 }
