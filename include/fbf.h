@@ -116,6 +116,26 @@ fbf_status fbf_search_host(fbf_dataset *ds, const float *queries, const float *m
                            uint64_t nq, uint32_t k, const fbf_search_opts *opts,
                            uint32_t *out_ids, float *out_dists);
 
+/* Captured batch (no counterpart in the reference; it replaces the router's
+ * per-batch brute-force loop, hybrid_graph.cpp:168-228, for a batch whose
+ * filters repeat).  fbf_search_graph_create plans the batch exactly as
+ * fbf_search does, uploads the plan into scratch owned by the graph, and
+ * captures the whole device sequence (threshold fills, prefix / main /
+ * rescan scans with their persistent work queues, merges and rerank) into
+ * one HIP graph.  fbf_search_graph_launch replays it on any stream: it reads
+ * queries_dev and writes out_ids_dev / out_dists_dev given at creation, so a
+ * caller may refill the query vectors between replays (the filters, metas,
+ * are fixed by the plan).  Replays on one graph must not overlap.  Searches
+ * on the dataset may run meanwhile (the graph has its own scratch).  The
+ * rescan count of fbf_last_search_rescans is not recorded for replays. */
+typedef struct fbf_search_graph fbf_search_graph;
+fbf_status fbf_search_graph_create(fbf_dataset *ds, const float *queries_dev, const float *metas,
+                                   uint64_t nq, uint32_t k, const fbf_search_opts *opts,
+                                   uint32_t *out_ids_dev, float *out_dists_dev, fbf_search_graph **out);
+fbf_status fbf_search_graph_launch(fbf_search_graph *g, void *stream);
+/* Waits for the device, then frees the graph and its scratch. */
+fbf_status fbf_search_graph_destroy(fbf_search_graph *g);
+
 /* Quantiser parameters of the dataset: alpha of the symmetric int8 codes
  * (SQ8SymmetricQuantizer::alpha, sq8_sym_quant.hpp:37-44) and the per-dim
  * min / (max - min) of the uint8 codes (SQ8Quantizer::mi/dif,

@@ -679,9 +679,18 @@ static fbf_status mode_check(const fbf_dataset *ds, uint32_t mode) {
     return FBF_OK;
 }
 
-extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, const float *metas,
-                                 uint64_t nq, uint32_t k, const fbf_search_opts *opts,
-                                 uint32_t *out_ids_dev, float *out_dists_dev, void *stream) {
+// A search's device work captured as one HIP graph (fbf_search_graph_create).
+struct GraphOut {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+// One search.  S: the scratch its plan and lists live in (the dataset's own,
+// shared by its searches, or a captured graph's).  cap: capture the device
+// work (everything after the plan upload) into a graph instead of running it.
+static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, const float *queries_dev,
+                              const float *metas, uint64_t nq, uint32_t k, const fbf_search_opts *opts,
+                              uint32_t *out_ids_dev, float *out_dists_dev, hipStream_t st, GraphOut *cap) {
     if (!ds || !metas || (nq && (!queries_dev || !out_ids_dev || !out_dists_dev)))
         return fail(FBF_ERR_INVALID_ARG, "null argument");
     if (k == 0 || k > 512) return fail(FBF_ERR_INVALID_ARG, "k must be in [1, 512]");
@@ -717,16 +726,19 @@ extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, cons
         if (mfma && persist) order_items_chunk_major(Q);
         else order_items(Q, !mfma);
     };
-    hipStream_t st = (hipStream_t)stream;
-
     std::lock_guard<std::mutex> lock(ds->mu);
     HIP_TRY(hipSetDevice(ds->device));
     const auto t_plan0 = std::chrono::steady_clock::now();
     // The dataset's scratch is reused by every search: a search on another
     // stream than the previous one first waits for that one.
-    if (ds->scr.used && ds->scr.last_stream != st) HIP_TRY(hipStreamSynchronize(ds->scr.last_stream));
-    ds->scr.used = true;
-    ds->scr.last_stream = st;
+    if (shared_scratch) {
+        if (S.used && S.last_stream != st) HIP_TRY(hipStreamSynchronize(S.last_stream));
+        S.used = true;
+        S.last_stream = st;
+    }
+    // Profiling events and the rescan count are host-side records of a run:
+    // neither goes into a captured graph.
+    const bool prof = ds->profile && cap == nullptr;
     std::vector<QRange> qr;
     s = compute_ranges(ds, metas, nq, qr);
     if (s != FBF_OK) return s;
@@ -792,10 +804,9 @@ extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, cons
     // is large (a scan whose threshold is still open then compacts rarely);
     // each of the two halves a lane appends to keeps room above ceil(K / 2) + 16.
     const int B = mfma ? (K <= 160 ? 256 : K <= 300 ? 512 : 1024) : list_capacity((int)K);
-    Scratch &S = ds->scr;
     s = upload_plan(S, P, nq, B, st);
     if (s != FBF_OK) return s;
-    if (ds->profile) {
+    if (prof) {
         ProfEvent ev;
         ev.kind = 4;
         ev.host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_plan0).count();
@@ -803,9 +814,29 @@ extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, cons
     }
     uint32_t *queue = nullptr;
     if (persist) queue = S.d_queue.as<uint32_t>();
-    // Per-query thresholds shared by all work items of a query.
+    // Every device buffer of the device work is reserved here, before it:
+    // a captured graph may not allocate.
     HIP_TRY(S.d_gtau.reserve(nq * sizeof(uint64_t)));
     uint64_t *gtau = S.d_gtau.as<uint64_t>();
+    if (seeded) {
+        HIP_TRY(S.d_tau0.reserve(nq * sizeof(uint64_t)));
+        HIP_TRY(S.d_flags.reserve((nq + 1) * sizeof(uint32_t)));
+        HIP_TRY(S.h_nflag.reserve(sizeof(uint32_t)));
+        if (!S.ev_nflag) HIP_TRY(hipEventCreateWithFlags(&S.ev_nflag, hipEventDisableTiming));
+    }
+    if (o.mode == FBF_MODE_SQ8SYM) HIP_TRY(S.d_q.reserve(nq * 112));
+    ds->last_seeded = false;
+    if (cap) {
+        // The plan is on the device once the stream drains; the launches
+        // below go into the graph.
+        if (mfma) warm_scan_mfma(o.mode == FBF_MODE_BF16 ? 0 : o.mode == FBF_MODE_FP8 ? 1 : 2, B,
+                                 gmax <= 64 ? 1 : gmax <= 128 ? 2 : 4);
+        HIP_TRY(hipStreamSynchronize(st));
+        HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    }
+    fbf_status sw = [&]() -> fbf_status {
+    fbf_status s = FBF_OK;
+    // Per-query thresholds shared by all work items of a query.
     HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(gtau), 0xFFFFFFFFu, 2 * nq, st));
 
     // Merge (and rerank) of the per-item lists into the outputs.
@@ -824,9 +855,9 @@ extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, cons
     };
     auto run_merge = [&](MergeArgs &m) -> fbf_status {
         ProfEvent ev;
-        record_begin(ds, exact ? 1 : 2, st, ev);
+        if (prof) record_begin(ds, exact ? 1 : 2, st, ev);
         HIP_TRY(launch_merge(m, st));
-        record_end(ds, st, ev);
+        if (prof) record_end(ds, st, ev);
         return FBF_OK;
     };
 
@@ -863,9 +894,9 @@ extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, cons
                 HIP_TRY(launch_fill_u32(queue, 0u, 8 * 16, st));
             }
             ProfEvent ev;
-            record_begin(ds, 0, st, ev);
+            if (prof) record_begin(ds, 0, st, ev);
             HIP_TRY(launch_scan_mfma(variant, a, B, nw, st));
-            record_end(ds, st, ev);
+            if (prof) record_end(ds, st, ev);
             return FBF_OK;
         };
         if (seeded) {
@@ -895,14 +926,14 @@ extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, cons
             m.pass = 1;
             s = run_merge(m);
             if (s != FBF_OK) return s;
-            HIP_TRY(S.h_nflag.reserve(sizeof(uint32_t)));
-            if (!S.ev_nflag) HIP_TRY(hipEventCreateWithFlags(&S.ev_nflag, hipEventDisableTiming));
-            HIP_TRY(hipMemcpyAsync(S.h_nflag.p, flags + nq, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-            HIP_TRY(hipEventRecord(S.ev_nflag, st));
+            if (!cap) {
+                HIP_TRY(hipMemcpyAsync(S.h_nflag.p, flags + nq, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+                HIP_TRY(hipEventRecord(S.ev_nflag, st));
+            }
             s = scan(0, P.items.size(), K, flags);
             if (s != FBF_OK) return s;
             m.pass = 2;
-            ds->last_seeded = true;
+            ds->last_seeded = !cap && shared_scratch;
             return run_merge(m);
         }
         s = scan(0, P.items.size(), K, nullptr);
@@ -927,7 +958,7 @@ extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, cons
             const uint32_t L = P.items[i0].layout;
             if (queue) HIP_TRY(launch_fill_u32(queue, 0u, 1, st));
             ProfEvent ev;
-            record_begin(ds, 0, st, ev);
+            if (prof) record_begin(ds, 0, st, ev);
             if (exact) {
                 ScanArgs a{};
                 a.items = S.d_items.as<WorkItem>() + i0;
@@ -957,13 +988,100 @@ extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, cons
                 a.queue = queue;
                 HIP_TRY(launch_scan_sq8sym(a, B, st));
             }
-            record_end(ds, st, ev);
+            if (prof) record_end(ds, st, ev);
         }
     }
 
     MergeArgs m = make_merge();
-    ds->last_seeded = false;
     return run_merge(m);
+    }();
+    if (cap) {
+        // End the capture on every path (a stream left capturing is unusable).
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(st, &g);
+        if (sw != FBF_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return sw;
+        }
+        if (e != hipSuccess || g == nullptr) {
+            if (g) (void)hipGraphDestroy(g);
+            return fail(FBF_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        }
+        hipGraphExec_t x = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        if (ei != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            return fail(FBF_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+        }
+        cap->graph = g;
+        cap->exec = x;
+    }
+    return sw;
+}
+
+extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, const float *metas,
+                                 uint64_t nq, uint32_t k, const fbf_search_opts *opts,
+                                 uint32_t *out_ids_dev, float *out_dists_dev, void *stream) {
+    if (!ds) return fail(FBF_ERR_INVALID_ARG, "null argument");
+    return search_impl(ds, ds->scr, true, queries_dev, metas, nq, k, opts, out_ids_dev, out_dists_dev,
+                       (hipStream_t)stream, nullptr);
+}
+
+// ---------------------------------------------------------------- captured searches
+struct fbf_search_graph {
+    fbf_dataset *ds = nullptr;
+    Scratch scr;      // the graph's own plan, lists and thresholds
+    GraphOut g;
+    uint64_t nq = 0;
+};
+
+extern "C" fbf_status fbf_search_graph_create(fbf_dataset *ds, const float *queries_dev, const float *metas,
+                                              uint64_t nq, uint32_t k, const fbf_search_opts *opts,
+                                              uint32_t *out_ids_dev, float *out_dists_dev,
+                                              fbf_search_graph **out) {
+    if (!ds || !out) return fail(FBF_ERR_INVALID_ARG, "null argument");
+    if (nq == 0) return fail(FBF_ERR_INVALID_ARG, "a captured search needs at least one query");
+    *out = nullptr;
+    HIP_TRY(hipSetDevice(ds->device));
+    auto *G = new (std::nothrow) fbf_search_graph();
+    if (!G) return fail(FBF_ERR_DEVICE, "out of host memory");
+    G->ds = ds;
+    G->nq = nq;
+    hipStream_t st = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    fbf_status s = e == hipSuccess ? FBF_OK : fail(FBF_ERR_DEVICE, "hipStreamCreate");
+    if (s == FBF_OK)
+        s = search_impl(ds, G->scr, false, queries_dev, metas, nq, k, opts, out_ids_dev, out_dists_dev, st, &G->g);
+    if (st) {
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamDestroy(st);
+    }
+    if (s != FBF_OK) {
+        fbf_search_graph_destroy(G);
+        return s;
+    }
+    *out = G;
+    return FBF_OK;
+}
+
+extern "C" fbf_status fbf_search_graph_launch(fbf_search_graph *g, void *stream) {
+    if (!g || !g->g.exec) return fail(FBF_ERR_INVALID_ARG, "null graph");
+    HIP_TRY(hipSetDevice(g->ds->device));
+    HIP_TRY(hipGraphLaunch(g->g.exec, (hipStream_t)stream));
+    return FBF_OK;
+}
+
+extern "C" fbf_status fbf_search_graph_destroy(fbf_search_graph *g) {
+    if (!g) return FBF_OK;
+    (void)hipSetDevice(g->ds->device);
+    // The caller's launches must be complete; the device is drained so that
+    // freeing the graph's buffers cannot race a replay still in flight.
+    (void)hipDeviceSynchronize();
+    if (g->g.exec) (void)hipGraphExecDestroy(g->g.exec);
+    if (g->g.graph) (void)hipGraphDestroy(g->g.graph);
+    g->scr.release();
+    delete g;
+    return FBF_OK;
 }
 
 extern "C" fbf_status fbf_search_host(fbf_dataset *ds, const float *queries, const float *metas,

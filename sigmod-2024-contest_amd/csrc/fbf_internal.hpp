@@ -171,6 +171,9 @@ hipError_t launch_scan_sq8sym(const ScanSymArgs &a, int B, hipStream_t st);
 // variant: 0 BF16, 1 FP8, 2 SQ8; nw: waves per workgroup (1, 2, 4), an item
 // carries at most 64 * nw queries
 hipError_t launch_scan_mfma(int variant, const ScanMfmaArgs &a, int B, int nw, hipStream_t st);
+// Resolve (and cache) the persistent grid of a scan instantiation ahead of a
+// stream capture.
+void warm_scan_mfma(int variant, int B, int nw);
 hipError_t launch_build_mfma(int variant, const float *rows, const uint32_t *perm, const float *norms,
                              const uint8_t *sq8_codes, uint64_t n, void *out, uint8_t *scale, hipStream_t st);
 hipError_t launch_row_norms(const float *rows, uint64_t n, float *out, hipStream_t st);

@@ -887,6 +887,11 @@ static int persistent_per_cu(int variant, int B, int nw) {
     return v;
 }
 
+void warm_scan_mfma(int variant, int B, int nw) {
+    if ((variant >= 0 && variant <= 2) && (B == 256 || B == 512 || B == 1024) && (nw == 1 || nw == 2 || nw == 4))
+        (void)persistent_per_cu(variant, B, nw);
+}
+
 // nw: waves per workgroup (1, 2 or 4); an item carries at most 64 * nw queries.
 hipError_t launch_scan_mfma(int variant, const ScanMfmaArgs &a, int B, int nw, hipStream_t st) {
     if (a.n_items == 0) return hipSuccess;

@@ -38,7 +38,8 @@ EXPORTS = [
     "fbf_search", "fbf_search_host", "fbf_dataset_quant_params", "fbf_register_codes",
     "fbf_unregister_codes", "fbf_bruteforce", "fbf_bruteforce_subgraph", "fbf_bruteforce_batch",
     "fbf_profile_enable", "fbf_profile_collect", "fbf_last_search_stats", "fbf_last_search_rescans",
-    "fbf_last_error", "fbf_version",
+    "fbf_last_error", "fbf_version", "fbf_search_graph_create", "fbf_search_graph_launch",
+    "fbf_search_graph_destroy",
 ]
 
 
@@ -85,6 +86,10 @@ def lib() -> C.CDLL:
     L.fbf_last_search_stats.argtypes = [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                         C.POINTER(C.c_uint64)]
     L.fbf_last_search_rescans.argtypes = [_P, C.POINTER(C.c_uint64)]
+    L.fbf_search_graph_create.argtypes = [_P, _P, _P, C.c_uint64, C.c_uint32, C.POINTER(SearchOpts), _P, _P,
+                                          C.POINTER(_P)]
+    L.fbf_search_graph_launch.argtypes = [_P, _P]
+    L.fbf_search_graph_destroy.argtypes = [_P]
     for name in EXPORTS:
         if name not in ("fbf_last_error", "fbf_version"):
             getattr(L, name).restype = C.c_int
@@ -275,3 +280,44 @@ def bruteforce_batch(codes_base: np.ndarray, starts, counts, qcodes: np.ndarray,
     _check(lib().fbf_bruteforce_batch(_np_ptr(codes_base), _np_ptr(starts), _np_ptr(counts), _np_ptr(qcodes),
                                       nq, k, _np_ptr(pos), _np_ptr(dist)), "fbf_bruteforce_batch")
     return pos, dist
+
+
+class SearchGraph:
+    """One batch's search captured as a HIP graph (fbf_search_graph_create):
+    ``launch()`` replays it, reading the query tensor given at creation (its
+    contents may change between replays; the filters may not) and writing
+    ``ids`` / ``dists``."""
+
+    def __init__(self, ds: "Dataset", queries, metas: np.ndarray, k: int = 100, mode="bf16", kprime: int = 0,
+                 chunk_rows: int = 0, group_queries: int = 0, persistent: int = 0, prepass_div: int = 0,
+                 prepass_margin: int = 0):
+        import torch
+        metas = np.ascontiguousarray(metas, np.float32)
+        nq = metas.shape[0]
+        assert queries.shape == (nq, ds.d) and queries.dtype == torch.float32 and queries.is_contiguous()
+        self.queries = queries  # kept alive: the graph reads it
+        self.ids = torch.empty((nq, k), dtype=torch.int32, device=queries.device)
+        self.dists = torch.empty((nq, k), dtype=torch.float32, device=queries.device)
+        o = Dataset._opts(mode, kprime, chunk_rows, group_queries, persistent, prepass_div, prepass_margin)
+        h = _P()
+        _check(lib().fbf_search_graph_create(ds.h, _P(_dev_ptr(queries)), _np_ptr(metas), nq, k, C.byref(o),
+                                             _P(_dev_ptr(self.ids)), _P(_dev_ptr(self.dists)), C.byref(h)),
+               "fbf_search_graph_create")
+        self.h = h
+
+    def launch(self, stream=None):
+        import torch
+        st = (stream or torch.cuda.current_stream(self.queries.device)).cuda_stream
+        _check(lib().fbf_search_graph_launch(self.h, _P(st)), "fbf_search_graph_launch")
+        return self.ids, self.dists
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            _check(lib().fbf_search_graph_destroy(self.h), "fbf_search_graph_destroy")
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
