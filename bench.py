@@ -52,6 +52,9 @@ def parse():
     p.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True,
                    help="pick group_queries / chunk_rows / kprime with the project's autotuner (untimed) "
                         "unless they are given (approximate modes)")
+    p.add_argument("--graph", action=argparse.BooleanOptionalAction, default=None,
+                   help="also time replays of the batch captured as one HIP graph (fbf_search_graph_*); "
+                        "default: on for the approximate modes")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                    help="only run a plain device copy of MIB MiB (HBM counter calibration) and exit")
     return p.parse_args()
@@ -163,6 +166,28 @@ def main() -> int:
     ids = out[0].cpu().numpy().view(np.uint32)
     dists = out[1].cpu().numpy()
 
+    # The same batch captured once as a HIP graph (planning and upload
+    # untimed), then replayed: the device work alone, with no host planning.
+    graph_qps = None
+    if a.graph or (a.graph is None and mode != "exact"):
+        kwg = {key: kw[key] for key in ("k", "mode", "kprime", "chunk_rows", "group_queries", "persistent",
+                                       "prepass_div", "prepass_margin")}
+        g = pkg.fbf.SearchGraph(ds, qd, metas, **kwg)
+        try:
+            g.launch()
+            torch.cuda.synchronize()
+            same = bool(np.array_equal(g.ids.cpu().numpy().view(np.uint32), ids))
+            t0 = time.perf_counter()
+            for i in range(a.steps):
+                g.launch()
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+        finally:
+            g.close()
+        graph_qps = nq * a.steps / dt
+        log(f"graph replays: {a.steps} in {dt:.3f}s: {graph_qps:.0f} QPS ({1e3 * dt / a.steps:.1f} ms/step), "
+            f"ids equal to the eager search: {same}")
+
     # Recall@k against FP64 exact on a type-stratified sample (CPU oracle).
     orc = ge.load_oracle()
     rng = np.random.default_rng(0)
@@ -220,7 +245,7 @@ def main() -> int:
         "data": "synthetic (seeded Gaussian mixture, Zipf categories, uniform timestamps)",
         "config": {"workload": cfg.name, "n": n, "d": 100, "nq": nq, "k": a.k, "mode": mode,
                    "kprime": kw["kprime"], "group_queries": kw["group_queries"], "chunk_rows": kw["chunk_rows"],
-                   "autotuned": tuned, "per_batch_searches": per_batch,
+                   "autotuned": tuned, "per_batch_searches": per_batch, "graph_replay_qps": graph_qps,
                    "pairs_per_step": pairs, "work_items": stats["work_items"]},
         "recall_at_k_vs_fp64": rec,
         "recall_sample": int(sample.size),
