@@ -91,9 +91,11 @@ struct LayoutDev {
     void *sq8 = nullptr;          // uint8 codes in MFMA order
     float *sq8_norm = nullptr;    // [n_pad] |xhat|^2 per position
     int8_t *sym = nullptr;        // [n_pad][112] symmetric int8 codes
+    void *sym_mfma = nullptr;     // symmetric int8 codes in MFMA order (K = 128)
+    float *sym_norm = nullptr;    // [n_pad] |codes|^2 per position
     void free_all() {
         for (void *p : {(void *)ids, (void *)soa, bf16, fp8, (void *)fp8_scale, (void *)xnorm, sq8,
-                        (void *)sq8_norm, (void *)sym})
+                        (void *)sq8_norm, (void *)sym, sym_mfma, (void *)sym_norm})
             if (p) (void)hipFree(p);
         *this = LayoutDev();
     }
@@ -315,6 +317,11 @@ static fbf_status dataset_build(fbf_dataset *ds, const float *labels, const floa
         if (repr & FBF_REPR_SQ8SYM) {
             HIP_TRY(hipMalloc(&D.sym, n_pad * 112));
             HIP_TRY(launch_sym_encode(ds->base_rows, D.ids, n, n_pad, ds->alpha, D.sym, st));
+            // The same codes for the int8 matrix cores (batched searches).
+            HIP_TRY(hipMalloc(&D.sym_mfma, nb32 * 4 * 64 * 16));
+            HIP_TRY(hipMalloc(&D.sym_norm, n_pad * sizeof(float)));
+            HIP_TRY(launch_build_mfma(3, nullptr, D.ids, D.sym_norm, reinterpret_cast<const uint8_t *>(D.sym), n,
+                                      D.sym_mfma, nullptr, st));
         }
     }
     record_end(ds, st, ev);
@@ -700,7 +707,8 @@ static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, 
     fbf_status s = mode_check(ds, o.mode);
     if (s != FBF_OK) return s;
     const bool exact = o.mode == FBF_MODE_EXACT_F32;
-    const bool mfma = o.mode == FBF_MODE_BF16 || o.mode == FBF_MODE_FP8 || o.mode == FBF_MODE_SQ8;
+    const bool mfma = o.mode == FBF_MODE_BF16 || o.mode == FBF_MODE_FP8 || o.mode == FBF_MODE_SQ8 ||
+                      o.mode == FBF_MODE_SQ8SYM;
     // Kept depth per list: k for the exact scan, k' (over-fetch) otherwise.
     uint32_t K = k;
     if (!exact) {
@@ -829,7 +837,7 @@ static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, 
     if (cap) {
         // The plan is on the device once the stream drains; the launches
         // below go into the graph.
-        if (mfma) warm_scan_mfma(o.mode == FBF_MODE_BF16 ? 0 : o.mode == FBF_MODE_FP8 ? 1 : 2, B,
+        if (mfma) warm_scan_mfma(o.mode == FBF_MODE_BF16 ? 0 : o.mode == FBF_MODE_FP8 ? 1 : o.mode == FBF_MODE_SQ8 ? 2 : 3, B,
                                  gmax <= 64 ? 1 : gmax <= 128 ? 2 : 4);
         HIP_TRY(hipStreamSynchronize(st));
         HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -864,8 +872,13 @@ static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, 
     // Scan.  Each work item writes every list slot it owns (count and keys),
     // so no clearing is needed.
     if (mfma) {
-        const int variant = o.mode == FBF_MODE_BF16 ? 0 : o.mode == FBF_MODE_FP8 ? 1 : 2;
+        const int variant = o.mode == FBF_MODE_BF16 ? 0 : o.mode == FBF_MODE_FP8 ? 1 : o.mode == FBF_MODE_SQ8 ? 2 : 3;
         ScanMfmaArgs a{};
+        if (variant == 3) {
+            // Query codes with the dataset's scale (SQ8SymmetricQuantizer::encode).
+            a.qcodes = S.d_q.as<int8_t>();
+            HIP_TRY(launch_sym_encode(queries_dev, nullptr, nq, nq, ds->alpha, S.d_q.as<int8_t>(), st));
+        }
         a.items = S.d_items.as<WorkItem>();
         a.n_items = P.items.size();
         a.iq = S.d_iq.as<ItemQuery>();
@@ -874,8 +887,8 @@ static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, 
         for (int L = 0; L < 2; ++L) {
             const LayoutDev &D = ds->lay[L];
             a.lay[L].ids = D.ids;
-            a.lay[L].tiles = variant == 0 ? D.bf16 : variant == 1 ? D.fp8 : D.sq8;
-            a.lay[L].xnorm = variant == 1 ? D.xnorm : D.sq8_norm;
+            a.lay[L].tiles = variant == 0 ? D.bf16 : variant == 1 ? D.fp8 : variant == 2 ? D.sq8 : D.sym_mfma;
+            a.lay[L].xnorm = variant == 1 ? D.xnorm : variant == 2 ? D.sq8_norm : D.sym_norm;
             a.lay[L].scale = D.fp8_scale;
         }
         a.lists = S.d_lists.as<uint64_t>();
@@ -939,13 +952,7 @@ static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, 
         s = scan(0, P.items.size(), K, nullptr);
         if (s != FBF_OK) return s;
     } else {
-        int8_t *qcodes = nullptr;
-        if (o.mode == FBF_MODE_SQ8SYM) {
-            HIP_TRY(S.d_q.reserve(nq * 112));
-            qcodes = S.d_q.as<int8_t>();
-            HIP_TRY(launch_sym_encode(queries_dev, nullptr, nq, nq, ds->alpha, qcodes, st));
-        }
-        // Runs of equal layout in item order (per pass: each pass is sorted by layout).
+        // Exact FP32 scan.  Runs of equal layout in item order (per pass: each pass is sorted by layout).
         std::vector<std::pair<uint64_t, uint64_t>> runs;
         for (uint64_t i = 0; i < P.items.size();) {
             uint64_t j = i + 1;
@@ -959,35 +966,19 @@ static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, 
             if (queue) HIP_TRY(launch_fill_u32(queue, 0u, 1, st));
             ProfEvent ev;
             if (prof) record_begin(ds, 0, st, ev);
-            if (exact) {
-                ScanArgs a{};
-                a.items = S.d_items.as<WorkItem>() + i0;
-                a.n_items = i1 - i0;
-                a.iq = S.d_iq.as<ItemQuery>();
-                a.queries = queries_dev;
-                a.soa = ds->lay[L].soa;
-                a.ids = ds->lay[L].ids;
-                a.lists = S.d_lists.as<uint64_t>();
-                a.counts = S.d_counts.as<uint32_t>();
-                a.gtau = gtau;
-                a.K = K;
-                a.queue = queue;
-                HIP_TRY(launch_scan_f32(a, B, st));
-            } else {
-                ScanSymArgs a{};
-                a.items = S.d_items.as<WorkItem>() + i0;
-                a.n_items = i1 - i0;
-                a.iq = S.d_iq.as<ItemQuery>();
-                a.qcodes = qcodes;
-                a.codes = ds->lay[L].sym;
-                a.ids = ds->lay[L].ids;
-                a.n_rows = ds->n;
-                a.lists = S.d_lists.as<uint64_t>();
-                a.counts = S.d_counts.as<uint32_t>();
-                a.K = K;
-                a.queue = queue;
-                HIP_TRY(launch_scan_sq8sym(a, B, st));
-            }
+            ScanArgs a{};
+            a.items = S.d_items.as<WorkItem>() + i0;
+            a.n_items = i1 - i0;
+            a.iq = S.d_iq.as<ItemQuery>();
+            a.queries = queries_dev;
+            a.soa = ds->lay[L].soa;
+            a.ids = ds->lay[L].ids;
+            a.lists = S.d_lists.as<uint64_t>();
+            a.counts = S.d_counts.as<uint32_t>();
+            a.gtau = gtau;
+            a.K = K;
+            a.queue = queue;
+            HIP_TRY(launch_scan_f32(a, B, st));
             if (prof) record_end(ds, st, ev);
         }
     }

@@ -110,7 +110,7 @@ struct ScanSymArgs {
 // One sorted layout as seen by the MFMA scans.
 struct LayoutMfma {
     const void *tiles;       // row fragments in MFMA order (variant encoding)
-    const float *xnorm;      // [n_pad32] |x|^2 (FP8) or |xhat|^2 (SQ8) per position
+    const float *xnorm;      // [n_pad32] |x|^2 (FP8), |xhat|^2 (SQ8) or |codes|^2 (SYM) per position
     const uint8_t *scale;    // [n_pad32] E8M0 row scale (FP8)
     const uint32_t *ids;     // position -> original id
 };
@@ -120,6 +120,7 @@ struct ScanMfmaArgs {
     uint64_t n_items;
     const ItemQuery *iq;
     const float *queries;    // [nq][kDim] FP32
+    const int8_t *qcodes;    // [nq][112] symmetric int8 query codes (SYM)
     const float *sq8_w;      // [kDim] dif_k / 255 (SQ8)
     LayoutMfma lay[2];
     uint64_t *lists;
@@ -168,8 +169,10 @@ hipError_t launch_scan_f32(const ScanArgs &a, int B, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st);
 hipError_t launch_merge(const MergeArgs &a, hipStream_t st);
 hipError_t launch_scan_sq8sym(const ScanSymArgs &a, int B, hipStream_t st);
-// variant: 0 BF16, 1 FP8, 2 SQ8; nw: waves per workgroup (1, 2, 4), an item
-// carries at most 64 * nw queries
+// variant: 0 BF16, 1 FP8, 2 SQ8, 3 symmetric int8; nw: waves per workgroup
+// (1, 2, 4), an item carries at most 64 * nw queries.  launch_build_mfma with
+// variant 3 reads the layout's symmetric codes from sq8_codes and writes the
+// code norms to norms.
 hipError_t launch_scan_mfma(int variant, const ScanMfmaArgs &a, int B, int nw, hipStream_t st);
 // Resolve (and cache) the persistent grid of a scan instantiation ahead of a
 // stream capture.

@@ -19,6 +19,10 @@
 //         dequantised in registers to bf16 (codes 0..255 are exact in bf16);
 //         the query side carries dif_i / 255; |xhat|^2 rides in the spare K
 //         slots as for BF16: score = acc.
+//   SYM:  the symmetric int8 codes (SQ8SymmetricQuantizer, sq8_sym_quant.hpp)
+//         on the int8 matrix cores, K = 128 (112 code bytes, zero beyond):
+//         score = |x|^2 - 2 x.q, integer-exact (|score| < 2^24, so exact in
+//         FP32); + |q|^2 gives L2SqrSQ8_sym (distance.hpp:329-366).
 //
 // MFMA lane layouts (measured on gfx950):
 //   v_mfma_f32_32x32x16_bf16: A lane l holds row l%32, k = 8*(l/32) + e;
@@ -26,6 +30,8 @@
 //     row 8*(j/4) + 4*(l/32) + j%4, column l%32.
 //   v_mfma_scale_f32_32x32x64_f8f6f4 (e4m3): lane l holds row / column l%32,
 //     k = 32*(l/32) + e, e = 0..31; same D layout.
+//   v_mfma_i32_32x32x32_i8: lane l holds row / column l%32, k = 16*(l/32) + e,
+//     e = 0..15; same D layout (int32).
 #include <atomic>
 #include <utility>
 
@@ -36,10 +42,14 @@ namespace fbf {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef int i32x8 __attribute__((ext_vector_type(8)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kMTile = 32;     // rows per MFMA tile
 constexpr int kBfSteps = 7;    // K = 112 = 7 x 16 (100 dims + 2 norm slots + pad)
 constexpr int kF8Steps = 2;    // K = 128 = 2 x 64
+constexpr int kI8Steps = 4;    // K = 128 = 4 x 32
+constexpr int kSymBytes = 112; // symmetric int8 code bytes per row (utils: 100 dims + 12 zero)
 
 __device__ inline uint16_t bf16_bits(float f) {  // round to nearest even
     uint32_t u = __builtin_bit_cast(uint32_t, f);
@@ -154,6 +164,35 @@ __global__ void build_sq8_kernel(const uint8_t *__restrict__ codes_orig, const u
     out[t] = make_uint2(lo, hi);
 }
 
+// Symmetric int8 tiles: block rb, step s (K 32), lane l: 16 bytes = codes of
+// row rb*32 + l%32 (layout order, 112-byte rows) at k = 32 s + 16 (l/32) + e,
+// zero beyond the 112 code bytes; xn[p] = |x|^2 of the codes (exact integer).
+__global__ void build_sym_kernel(const int8_t *__restrict__ codes, uint64_t n, uint64_t n_blocks,
+                                 uint4 *__restrict__ out, float *__restrict__ xn) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_blocks * kI8Steps * 64) return;
+    const uint32_t l = t % 64, s = (t / 64) % kI8Steps;
+    const uint64_t rb = t / (64 * kI8Steps);
+    const uint64_t p = rb * kMTile + l % 32;
+    uint32_t w[4] = {0, 0, 0, 0};
+    if (p < n) {
+        const int8_t *c = codes + p * kSymBytes;
+        for (int e = 0; e < 16; ++e) {
+            const int k = 32 * s + 16 * (l / 32) + e;
+            const uint32_t v = k < kSymBytes ? (uint32_t)(uint8_t)c[k] : 0u;
+            w[e / 4] |= v << (8 * (e % 4));
+        }
+        if (s == 0 && l < 32) {
+            int32_t sq = 0;
+            for (int k = 0; k < kSymBytes; ++k) sq += (int32_t)c[k] * (int32_t)c[k];
+            xn[p] = (float)sq;
+        }
+    } else if (s == 0 && l < 32) {
+        xn[p] = 0.f;
+    }
+    out[t] = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
 // Per-position FP32 values gathered into a layout (row norms).
 __global__ void gather_f32_kernel(const float *__restrict__ src, const uint32_t *__restrict__ perm,
                                   uint64_t n, uint64_t n_pad, float *__restrict__ dst) {
@@ -240,6 +279,15 @@ hipError_t launch_build_mfma(int variant, const float *rows, const uint32_t *per
     if (variant == 0 && (rows == nullptr || norms == nullptr)) return hipErrorInvalidValue;
     if (variant == 1 && (rows == nullptr || scale == nullptr)) return hipErrorInvalidValue;
     if (variant == 2 && (sq8_codes == nullptr || norms == nullptr)) return hipErrorInvalidValue;
+    if (variant == 3) {
+        // sq8_codes: the layout's symmetric int8 codes; norms: out, [n_blocks * 32]
+        if (sq8_codes == nullptr || norms == nullptr) return hipErrorInvalidValue;
+        const uint64_t total = n_blocks * kI8Steps * 64;
+        hipLaunchKernelGGL(build_sym_kernel, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, st,
+                           reinterpret_cast<const int8_t *>(sq8_codes), n, n_blocks, (uint4 *)out,
+                           const_cast<float *>(norms));
+        return hipGetLastError();
+    }
     if (variant == 0) {
         const uint64_t total = n_blocks * kBfSteps * 64;
         hipLaunchKernelGGL(build_bf16_kernel, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, st, rows,
@@ -291,10 +339,12 @@ template <int VAR> struct RowFrag;
 template <> struct RowFrag<0> { bf16x8 v[kBfSteps]; };                 // BF16
 template <> struct RowFrag<1> { i32x8 v[kF8Steps]; int scale; };      // FP8
 template <> struct RowFrag<2> { bf16x8 v[kBfSteps]; };                 // SQ8 (dequantised)
+template <> struct RowFrag<3> { i32x4 v[kI8Steps]; };                  // SYM int8
 template <int VAR> struct QFrag;
 template <> struct QFrag<0> { bf16x8 v[kBfSteps]; };
 template <> struct QFrag<1> { i32x8 v[kF8Steps]; int scale; };
 template <> struct QFrag<2> { bf16x8 v[kBfSteps]; };
+template <> struct QFrag<3> { i32x4 v[kI8Steps]; };
 
 __device__ inline bf16x8 bf16x8_from_bits(uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
     typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
@@ -313,6 +363,7 @@ template <int VAR> struct TileBytes;
 template <> struct TileBytes<0> { static constexpr int value = kBfSteps * 64 * 16; };
 template <> struct TileBytes<1> { static constexpr int value = kF8Steps * 64 * 32; };
 template <> struct TileBytes<2> { static constexpr int value = kBfSteps * 64 * 8; };
+template <> struct TileBytes<3> { static constexpr int value = kI8Steps * 64 * 16; };
 
 // This lane's A fragments of a tile staged in LDS (layout as in HBM).
 template <int VAR>
@@ -322,6 +373,12 @@ __device__ inline void frag_from_lds(const uint4 *sA, uint32_t lane, RowFrag<VAR
         for (int s = 0; s < kBfSteps; ++s) {
             const uint4 u = sA[s * 64 + lane];
             r.v[s] = bf16x8_from_bits(u.x, u.y, u.z, u.w);
+        }
+    } else if constexpr (VAR == 3) {
+#pragma unroll
+        for (int s = 0; s < kI8Steps; ++s) {
+            const uint4 u = sA[s * 64 + lane];
+            r.v[s] = i32x4{(int)u.x, (int)u.y, (int)u.z, (int)u.w};
         }
     } else if constexpr (VAR == 1) {
 #pragma unroll
@@ -377,6 +434,20 @@ __device__ inline void load_query(const ScanMfmaArgs &a, bool valid, uint32_t qi
             }
             f.v[s] = bf16x8_from_bits(w[0], w[1], w[2], w[3]);
         }
+    } else if constexpr (VAR == 3) {
+        // The query's symmetric int8 codes (a.qcodes, 112 bytes per query).
+        const int8_t *c = a.qcodes + (uint64_t)qi * kSymBytes;
+#pragma unroll
+        for (int s = 0; s < kI8Steps; ++s) {
+            uint32_t w[4] = {0, 0, 0, 0};
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const int k = 32 * s + 16 * half + e;
+                const uint32_t v = (valid && k < kSymBytes) ? (uint32_t)(uint8_t)c[k] : 0u;
+                w[e / 4] |= v << (8 * (e % 4));
+            }
+            f.v[s] = i32x4{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
+        }
     } else {
         float am = 0.f;
         if (valid)
@@ -405,7 +476,14 @@ __device__ inline void load_query(const ScanMfmaArgs &a, bool valid, uint32_t qi
 template <int VAR>
 __device__ inline f32x16 tile_mma(const RowFrag<VAR> &r, const QFrag<VAR> &q) {
     f32x16 acc = {};
-    if constexpr (VAR == 1) {
+    if constexpr (VAR == 3) {
+        i32x16 ia = {};
+#pragma unroll
+        for (int s = 0; s < kI8Steps; ++s) ia = __builtin_amdgcn_mfma_i32_32x32x32_i8(r.v[s], q.v[s], ia, 0, 0, 0);
+        // -2 x.q: exact in FP32 (|x.q| <= 112 * 127^2 < 2^21)
+#pragma unroll
+        for (int j = 0; j < 16; ++j) acc[j] = -2.0f * (float)ia[j];
+    } else if constexpr (VAR == 1) {
 #pragma unroll
         for (int s = 0; s < kF8Steps; ++s)
             acc = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(r.v[s], q.v[s], acc, 0, 0, 0, r.scale, 0, q.scale);
@@ -637,13 +715,14 @@ _Pragma("unroll")                                                               
             hit[b] = __ballot(live && qv[b] && t < qe[b] && t + kMTile > qs[b]);                      \
             hit_any |= hit[b] != 0;                                                                   \
         }                                                                                             \
-/* FP8 row scales and norms: loaded before the tile's direct loads are */                            \
-/* issued, so waiting for them does not wait for the tile.              */                            \
+/* FP8 row scales, FP8 / SYM row norms: loaded before the tile's      */                            \
+/* direct loads are issued, so waiting for them does not wait for the   */                            \
+/* tile.                                                                 */                           \
         float xn[16];                                                                                 \
         int rscale = 0;                                                                               \
-        if constexpr (VAR == 1) {                                                                     \
+        if constexpr (VAR == 1 || VAR == 3) {                                                         \
             if (hit_any) {                                                                            \
-                rscale = (int)L.scale[t + col] * 0x01010101;                                          \
+                if constexpr (VAR == 1) rscale = (int)L.scale[t + col] * 0x01010101;                  \
 _Pragma("unroll")                                                                                     \
                 for (int g = 0; g < 4; ++g) {                                                         \
                     const float4 v = *reinterpret_cast<const float4 *>(L.xnorm + t + 8 * g + 4 * half); \
@@ -670,7 +749,7 @@ _Pragma("unroll")                                                               
                 f32x16 acc = tile_mma<VAR>(rf, qf[b]);                                                \
                 float sc[16];                                                                         \
 _Pragma("unroll")                                                                                     \
-                for (int j = 0; j < 16; ++j) sc[j] = VAR == 1 ? acc[j] + xn[j] : acc[j];              \
+                for (int j = 0; j < 16; ++j) sc[j] = (VAR == 1 || VAR == 3) ? acc[j] + xn[j] : acc[j]; \
 /* One compare per lane: the smallest of its 16 scores against the    */                           \
 /* lane's threshold, or against -inf when the tile lies wholly outside */                            \
 /* the lane's query range (the block runs for the other queries).      */                            \
@@ -876,19 +955,22 @@ static int occupancy_var(int B, int nw) {
 
 // Resident workgroups per CU of a scan instantiation (cached per variant, B, nw).
 static int persistent_per_cu(int variant, int B, int nw) {
-    static std::atomic<int> cache[3][3][3] = {};
+    static std::atomic<int> cache[4][3][3] = {};
     const int bi = B == 256 ? 0 : B == 512 ? 1 : 2, wi = nw == 1 ? 0 : nw == 2 ? 1 : 2;
     std::atomic<int> &c = cache[variant][bi][wi];
     int v = c.load(std::memory_order_relaxed);
     if (v == 0) {
-        v = variant == 0 ? occupancy_var<0>(B, nw) : variant == 1 ? occupancy_var<1>(B, nw) : occupancy_var<2>(B, nw);
+        v = variant == 0   ? occupancy_var<0>(B, nw)
+            : variant == 1 ? occupancy_var<1>(B, nw)
+            : variant == 2 ? occupancy_var<2>(B, nw)
+                           : occupancy_var<3>(B, nw);
         c.store(v, std::memory_order_relaxed);
     }
     return v;
 }
 
 void warm_scan_mfma(int variant, int B, int nw) {
-    if ((variant >= 0 && variant <= 2) && (B == 256 || B == 512 || B == 1024) && (nw == 1 || nw == 2 || nw == 4))
+    if ((variant >= 0 && variant <= 3) && (B == 256 || B == 512 || B == 1024) && (nw == 1 || nw == 2 || nw == 4))
         (void)persistent_per_cu(variant, B, nw);
 }
 
@@ -909,7 +991,8 @@ hipError_t launch_scan_mfma(int variant, const ScanMfmaArgs &a, int B, int nw, h
     }
     if (variant == 0) return launch_mfma_var<0>(a, B, nw, grid, st);
     if (variant == 1) return launch_mfma_var<1>(a, B, nw, grid, st);
-    return launch_mfma_var<2>(a, B, nw, grid, st);
+    if (variant == 2) return launch_mfma_var<2>(a, B, nw, grid, st);
+    return launch_mfma_var<3>(a, B, nw, grid, st);
 }
 
 }  // namespace fbf

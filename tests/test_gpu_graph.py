@@ -28,7 +28,7 @@ def data(pkg):
     q, metas = dg.make_queries(cfg, nq=300)
     metas[:60, 0] = 0  # unfiltered: 200000 rows, seeded prefix pass
     q2 = (q + 0.3 * np.random.default_rng(7).standard_normal(q.shape)).astype(np.float32)
-    ds = fbf.Dataset(vecs, labels, ts, repr_flags=fbf.REPR_F32 | fbf.REPR_BF16 | fbf.REPR_SQ8)
+    ds = fbf.Dataset(vecs, labels, ts, repr_flags=fbf.REPR_F32 | fbf.REPR_BF16 | fbf.REPR_SQ8 | fbf.REPR_SQ8SYM)
     yield q, q2, metas, ds
     ds.close()
 
@@ -38,7 +38,7 @@ def _host(t):
 
 
 @pytest.mark.parametrize("mode,kw", [("bf16", {}), ("bf16", {"prepass_margin": 10}), ("sq8", {}),
-                                     ("exact", {})])
+                                     ("sq8sym", {}), ("exact", {})])
 def test_graph_replays_equal_eager(pkg, data, mode, kw):
     import torch
     fbf = pkg.fbf

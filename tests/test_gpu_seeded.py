@@ -14,7 +14,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-MODES = ["bf16", "sq8", "fp8"]
+MODES = ["bf16", "sq8", "fp8", "sq8sym"]
 
 
 def _say(msg):
@@ -31,7 +31,7 @@ def big(pkg):
     vecs, labels, ts = dg.make_base(cfg, n=200_000)
     q, metas = dg.make_queries(cfg, nq=160)
     metas[:40, 0] = 0  # unfiltered: 200000 rows, prefix 25000 rows at prepass_div 8
-    ds = fbf.Dataset(vecs, labels, ts, repr_flags=fbf.REPR_BF16 | fbf.REPR_SQ8 | fbf.REPR_FP8)
+    ds = fbf.Dataset(vecs, labels, ts, repr_flags=fbf.REPR_BF16 | fbf.REPR_SQ8 | fbf.REPR_FP8 | fbf.REPR_SQ8SYM)
     yield vecs, labels, ts, q, metas, ds
     ds.close()
 
