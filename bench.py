@@ -223,7 +223,8 @@ def main() -> int:
         pk = PEAK_TFLOPS["bf16"] if mode in ("bf16", "sq8") else PEAK_TFLOPS["fp8"] if mode == "fp8" else \
             PEAK_TFLOPS["int8"]
         achieved = flops / (avg_launch_ms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": achieved, "peak": pk, "unit": "TFLOP/s",
+        roof = {"bound": "mfma", "achieved": achieved, "peak": pk,
+                "unit": "TOP/s" if mode == "sq8sym" else "TFLOP/s",
                 "frac": achieved / pk, "traffic": committed_traffic(cfg.name, mode, a, kw)}
 
     cpu = None
