@@ -228,8 +228,10 @@ def main() -> int:
                 "frac": achieved / pk, "traffic": committed_traffic(cfg.name, mode, a, kw)}
 
     cpu = None
+    cpu_router = None
     if not a.no_cpu:
         cpu = cpu_baseline(orc, vecs, labels, ts, q, metas, a.k, a.cpu_seconds)
+        cpu_router = cpu_router_bf(orc, vecs, labels, ts, q, metas, a.k, a.cpu_seconds / 2)
 
     line = {
         "metric": "filtered top-100 QPS at recall@100>=0.99 (BF16/SQ8 vs FP32), 1 MI355X",
@@ -253,6 +255,7 @@ def main() -> int:
         "max_rel_err_vs_fp64": kth_err,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "cpu_baseline_router_bf": cpu_router,
     }
     print(json.dumps(line), flush=True)
     ds.close()
@@ -308,6 +311,45 @@ def cpu_baseline(orc, vecs, labels, ts, q, metas, k, target_s):
     return {"value": done / spent, "unit": "queries/s", "cores": threads, "kind": "port",
             "sample": f"{done} type-stratified queries of the same workload, exact canonical FP32 over "
                       f"sorted ranges (oracle/fbf_oracle.c orc_exact_ranges, OpenMP)"}
+
+
+def cpu_router_bf(orc, vecs, labels, ts, q, metas, k, target_s):
+    """The reference's own brute-force route on the CPU (hybrid_graph.cpp:
+    203-228 restated in the oracle, orc_router_bf): symmetric int8 codes,
+    top-140 scan of the filtered range, exact FP32 rerank to k, all host
+    threads, on a type-stratified sample sized to about target_s seconds."""
+    t0 = time.time()
+    L = orc.Layouts(labels.astype(np.int32), ts)
+    alpha = orc.sq8sym_train(vecs)
+    codes = orc.sq8sym_encode(vecs, alpha)
+    cb = np.ascontiguousarray(codes[L.perm_both])
+    ct = np.ascontiguousarray(codes[L.perm_time])
+    del codes
+    log(f"cpu router path: codes and layouts in {time.time() - t0:.1f}s")
+    rng = np.random.default_rng(2)
+    per = [rng.permutation(np.nonzero(metas[:, 0] == t)[0]) for t in range(4)]
+    m = min(len(p) for p in per)
+    order = np.stack([p[:m] for p in per], 1).reshape(-1)
+    done, spent, size = 0, 0.0, 4
+    n = vecs.shape[0]
+    while done < order.size and spent < target_s:
+        idx = np.sort(order[done:done + size])
+        qq = np.ascontiguousarray(q[idx])
+        mm = np.ascontiguousarray(metas[idx])
+        ids = np.zeros((idx.size, k), np.uint32)
+        dd = np.zeros((idx.size, k), np.float32)
+        t = time.perf_counter()
+        orc.lib().orc_router_bf(orc._p(cb), orc._p(ct), orc._p(L.perm_both), orc._p(L.lab_b), orc._p(L.ts_b),
+                                orc._p(L.perm_time), orc._p(L.ts_t), orc._p(vecs), n, 100, alpha,
+                                orc._p(qq), orc._p(mm), idx.size, 140, k, orc._p(ids), orc._p(dd))
+        spent += time.perf_counter() - t
+        done += idx.size
+        size = min(size * 2, 4096)
+    threads = orc.num_threads()
+    log(f"cpu router path (SQ8-sym top-140 + FP32 rerank): {done} queries in {spent:.1f}s on {threads} threads")
+    return {"value": done / spent, "unit": "queries/s", "cores": threads, "kind": "port",
+            "sample": f"{done} type-stratified queries; the reference's brute-force route (symmetric int8 "
+                      f"top-140 scan + exact FP32 rerank, oracle/fbf_oracle.c orc_router_bf, OpenMP)"}
 
 
 if __name__ == "__main__":
