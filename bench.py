@@ -265,7 +265,9 @@ def main() -> int:
 def committed_traffic(workload, mode, a, kw):
     """HBM bytes per scan launch measured with rocprofv3 --pmc (FETCH_SIZE +
     WRITE_SIZE, calibrated on a plain copy kernel) for this workload, mode and
-    search options, from the committed profile summary; None otherwise."""
+    search options, from the committed profile summary; None otherwise.  The
+    summary holds a `search_options:` / `traffic_bytes_per_launch:` pair for
+    its main run and may list more option sets as `traffic_for: {...} = B`."""
     if a.k != 100 or a.prepass_div or a.prepass_margin or a.n or a.nq:
         return None
     name = f"scan_f32_exact_{workload}.md" if mode == "exact" else f"scan_mfma_{mode}_{workload}.md"
@@ -280,6 +282,10 @@ def committed_traffic(workload, mode, a, kw):
                 traffic = float(line.split(":", 1)[1])
             if line.startswith("search_options:"):
                 profiled = json.loads(line.split(":", 1)[1])
+            if line.startswith("traffic_for:"):
+                spec, val = line.split(":", 1)[1].rsplit("=", 1)
+                if json.loads(spec) == opts:
+                    return float(val)
     return traffic if profiled == opts else None
 
 
