@@ -726,6 +726,11 @@ static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, 
     // waves share each row tile); VALU items up to 16.
     const uint32_t gcap = mfma ? 256u : (uint32_t)kMaxGroup;
     const uint32_t gmax = o.group_queries ? std::min<uint32_t>(o.group_queries, gcap) : mfma ? 128u : gcap;
+    // Prefix-pass items carry at most 128 queries (two waves per tile): the
+    // prefix lists start empty, so its waves compact often, and every
+    // compaction holds up the other waves that share the tile ring.
+    const uint32_t gmax_pre = std::min<uint32_t>(gmax, 128u);
+    auto waves_of = [](uint32_t g) { return g <= 64 ? 1 : g <= 128 ? 2 : 4; };
     if (o.persistent > 2) return fail(FBF_ERR_INVALID_ARG, "persistent must be 0, 1 or 2");
     const bool persist = o.persistent == 1 || (o.persistent == 0 && mfma);
     // Item order: chunk-major runs per XCD for the persistent MFMA scans, else
@@ -789,7 +794,7 @@ static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, 
                 make_plan(qb, chunk, gmax, PB);
                 order(PB);
             });
-            make_plan(qa, chunk, gmax, P);
+            make_plan(qa, chunk, gmax_pre, P);
             order(P);
             tb.join();
             n_items_a = P.items.size();
@@ -837,8 +842,11 @@ static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, 
     if (cap) {
         // The plan is on the device once the stream drains; the launches
         // below go into the graph.
-        if (mfma) warm_scan_mfma(o.mode == FBF_MODE_BF16 ? 0 : o.mode == FBF_MODE_FP8 ? 1 : o.mode == FBF_MODE_SQ8 ? 2 : 3, B,
-                                 gmax <= 64 ? 1 : gmax <= 128 ? 2 : 4);
+        if (mfma) {
+            const int v = o.mode == FBF_MODE_BF16 ? 0 : o.mode == FBF_MODE_FP8 ? 1 : o.mode == FBF_MODE_SQ8 ? 2 : 3;
+            warm_scan_mfma(v, B, waves_of(gmax));
+            if (seeded) warm_scan_mfma(v, B, waves_of(gmax_pre));
+        }
         HIP_TRY(hipStreamSynchronize(st));
         HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     }
@@ -894,8 +902,10 @@ static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, 
         a.lists = S.d_lists.as<uint64_t>();
         a.counts = S.d_counts.as<uint32_t>();
         a.gtau = gtau;
-        const int nw = gmax <= 64 ? 1 : gmax <= 128 ? 2 : 4;
+        // Waves per workgroup: enough for the largest item of the launch (a
+        // wave with no queries of an item only helps load its rows).
         auto scan = [&](uint64_t i0, uint64_t i1, uint32_t kk, const uint32_t *mask) -> fbf_status {
+            const int nw = waves_of(seeded && i1 <= n_items_a ? gmax_pre : gmax);
             if (i1 == i0) return FBF_OK;
             a.items = S.d_items.as<WorkItem>() + i0;
             a.n_items = i1 - i0;

@@ -37,7 +37,8 @@ def _host(t):
     return t.cpu().numpy()
 
 
-@pytest.mark.parametrize("mode,kw", [("bf16", {}), ("bf16", {"prepass_margin": 10}), ("sq8", {}),
+@pytest.mark.parametrize("mode,kw", [("bf16", {}), ("bf16", {"prepass_margin": 10}),
+                                     ("bf16", {"group_queries": 256}), ("sq8", {}),
                                      ("sq8sym", {}), ("exact", {})])
 def test_graph_replays_equal_eager(pkg, data, mode, kw):
     import torch

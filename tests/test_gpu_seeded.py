@@ -50,10 +50,12 @@ def test_seeded_equals_unseeded(pkg, big, mode):
     _say(f"case: N=200000 nq=160 mode={mode} unseeded reference")
     ref_i, ref_d = _search(ds, q, metas, k=100, mode=mode, prepass_div=1)
     assert ds.last_rescans() == 0
-    for div, margin, persistent in [(0, 0, 0), (8, 10, 0), (8, 10, 2), (4, 400, 1), (8, 0, 2)]:
-        _say(f"case: mode={mode} prepass_div={div} margin={margin} persistent={persistent}")
+    # group 256: the prefix pass runs 2-wave items, the main pass 4-wave items
+    for div, margin, persistent, group in [(0, 0, 0, 0), (8, 10, 0, 0), (8, 10, 2, 0), (4, 400, 1, 0), (8, 0, 2, 0),
+                                           (0, 0, 0, 256), (8, 10, 0, 256)]:
+        _say(f"case: mode={mode} prepass_div={div} margin={margin} persistent={persistent} group={group}")
         ids, d = _search(ds, q, metas, k=100, mode=mode, prepass_div=div, prepass_margin=margin,
-                         persistent=persistent)
+                         persistent=persistent, group_queries=group)
         resc = ds.last_rescans()
         _say(f"  rescanned {resc}")
         assert np.array_equal(ids, ref_i), (mode, div, margin)
