@@ -817,6 +817,10 @@ static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, 
     // is large (a scan whose threshold is still open then compacts rarely);
     // each of the two halves a lane appends to keeps room above ceil(K / 2) + 16.
     const int B = mfma ? (K <= 160 ? 256 : K <= 300 ? 512 : 1024) : list_capacity((int)K);
+    // The prefix pass keeps only kpre keys: with a small kpre its lists use
+    // 128 of their B slots, so each compaction sorts half as many keys and
+    // its threshold tightens after fewer appends.
+    const int B_pre = (seeded && kpre <= 64 && B >= 128) ? 128 : B;
     s = upload_plan(S, P, nq, B, st);
     if (s != FBF_OK) return s;
     if (prof) {
@@ -845,7 +849,7 @@ static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, 
         if (mfma) {
             const int v = o.mode == FBF_MODE_BF16 ? 0 : o.mode == FBF_MODE_FP8 ? 1 : o.mode == FBF_MODE_SQ8 ? 2 : 3;
             warm_scan_mfma(v, B, waves_of(gmax));
-            if (seeded) warm_scan_mfma(v, B, waves_of(gmax_pre));
+            if (seeded) warm_scan_mfma(v, B_pre, waves_of(gmax_pre));
         }
         HIP_TRY(hipStreamSynchronize(st));
         HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -902,10 +906,12 @@ static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, 
         a.lists = S.d_lists.as<uint64_t>();
         a.counts = S.d_counts.as<uint32_t>();
         a.gtau = gtau;
+        a.list_stride = (uint32_t)B;
         // Waves per workgroup: enough for the largest item of the launch (a
         // wave with no queries of an item only helps load its rows).
         auto scan = [&](uint64_t i0, uint64_t i1, uint32_t kk, const uint32_t *mask) -> fbf_status {
-            const int nw = waves_of(seeded && i1 <= n_items_a ? gmax_pre : gmax);
+            const bool pre = seeded && i1 <= n_items_a;
+            const int nw = waves_of(pre ? gmax_pre : gmax);
             if (i1 == i0) return FBF_OK;
             a.items = S.d_items.as<WorkItem>() + i0;
             a.n_items = i1 - i0;
@@ -918,7 +924,7 @@ static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, 
             }
             ProfEvent ev;
             if (prof) record_begin(ds, 0, st, ev);
-            HIP_TRY(launch_scan_mfma(variant, a, B, nw, st));
+            HIP_TRY(launch_scan_mfma(variant, a, pre ? B_pre : B, nw, st));
             if (prof) record_end(ds, st, ev);
             return FBF_OK;
         };

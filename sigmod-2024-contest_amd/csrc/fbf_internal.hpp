@@ -127,6 +127,7 @@ struct ScanMfmaArgs {
     uint32_t *counts;
     uint64_t *gtau;          // [nq] shared per-query threshold (key), tightened by every item
     uint32_t K;
+    uint32_t list_stride;    // keys between consecutive list slots (>= the launch's list capacity B)
     // Persistent form (queue != null): the items are split into 8 runs, run x
     // = [xoff[x], xoff[x + 1]), one per XCD; workgroups take items from their
     // own XCD's run (counter queue[16 x]) and then from the others' runs.
@@ -169,8 +170,9 @@ hipError_t launch_scan_f32(const ScanArgs &a, int B, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st);
 hipError_t launch_merge(const MergeArgs &a, hipStream_t st);
 hipError_t launch_scan_sq8sym(const ScanSymArgs &a, int B, hipStream_t st);
-// variant: 0 BF16, 1 FP8, 2 SQ8, 3 symmetric int8; nw: waves per workgroup
-// (1, 2, 4), an item carries at most 64 * nw queries.  launch_build_mfma with
+// variant: 0 BF16, 1 FP8, 2 SQ8, 3 symmetric int8; B: list capacity (128,
+// 256, 512, 1024; at most a.list_stride); nw: waves per workgroup (1, 2, 4),
+// an item carries at most 64 * nw queries.  launch_build_mfma with
 // variant 3 reads the layout's symmetric codes from sq8_codes and writes the
 // code norms to norms.
 hipError_t launch_scan_mfma(int variant, const ScanMfmaArgs &a, int B, int nw, hipStream_t st);

@@ -648,7 +648,7 @@ __device__ __attribute__((always_inline)) void scan_mfma_item(const ScanMfmaArgs
     const uint4 *gA = reinterpret_cast<const uint4 *>(L.tiles);
     auto qi_of = [&](uint32_t j) { return a.iq[W.q_begin + q0 + j].qi; };
     auto slot_of = [&](uint32_t j) { return list_slot(a.iq[W.q_begin + q0 + j], W); };
-    auto list_of = [&](uint32_t j) { return a.lists + slot_of(j) * (uint64_t)B; };
+    auto list_of = [&](uint32_t j) { return a.lists + slot_of(j) * (uint64_t)a.list_stride; };
 
     // Start from the thresholds earlier items (or the prefix pass) published.
     for (uint32_t j = lane; j < nq; j += 64)
@@ -923,6 +923,7 @@ void scan_mfma_kernel(ScanMfmaArgs a) {
 template <int VAR, int NW>
 static hipError_t launch_mfma_nw(const ScanMfmaArgs &a, int B, uint64_t grid, hipStream_t st) {
     switch (B) {
+    case 128: hipLaunchKernelGGL((scan_mfma_kernel<VAR, NW, 128>), dim3((uint32_t)grid), dim3(64 * NW), 0, st, a); break;
     case 256: hipLaunchKernelGGL((scan_mfma_kernel<VAR, NW, 256>), dim3((uint32_t)grid), dim3(64 * NW), 0, st, a); break;
     case 512: hipLaunchKernelGGL((scan_mfma_kernel<VAR, NW, 512>), dim3((uint32_t)grid), dim3(64 * NW), 0, st, a); break;
     case 1024: hipLaunchKernelGGL((scan_mfma_kernel<VAR, NW, 1024>), dim3((uint32_t)grid), dim3(64 * NW), 0, st, a); break;
@@ -942,6 +943,7 @@ template <int VAR, int NW>
 static int occupancy_nw(int B) {
     int n = 0;
     hipError_t e = hipErrorInvalidValue;
+    if (B == 128) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, scan_mfma_kernel<VAR, NW, 128>, 64 * NW, 0);
     if (B == 256) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, scan_mfma_kernel<VAR, NW, 256>, 64 * NW, 0);
     if (B == 512) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, scan_mfma_kernel<VAR, NW, 512>, 64 * NW, 0);
     if (B == 1024) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, scan_mfma_kernel<VAR, NW, 1024>, 64 * NW, 0);
@@ -955,8 +957,8 @@ static int occupancy_var(int B, int nw) {
 
 // Resident workgroups per CU of a scan instantiation (cached per variant, B, nw).
 static int persistent_per_cu(int variant, int B, int nw) {
-    static std::atomic<int> cache[4][3][3] = {};
-    const int bi = B == 256 ? 0 : B == 512 ? 1 : 2, wi = nw == 1 ? 0 : nw == 2 ? 1 : 2;
+    static std::atomic<int> cache[4][4][3] = {};
+    const int bi = B == 128 ? 0 : B == 256 ? 1 : B == 512 ? 2 : 3, wi = nw == 1 ? 0 : nw == 2 ? 1 : 2;
     std::atomic<int> &c = cache[variant][bi][wi];
     int v = c.load(std::memory_order_relaxed);
     if (v == 0) {
@@ -970,7 +972,7 @@ static int persistent_per_cu(int variant, int B, int nw) {
 }
 
 void warm_scan_mfma(int variant, int B, int nw) {
-    if ((variant >= 0 && variant <= 3) && (B == 256 || B == 512 || B == 1024) && (nw == 1 || nw == 2 || nw == 4))
+    if ((variant >= 0 && variant <= 3) && (B == 128 || B == 256 || B == 512 || B == 1024) && (nw == 1 || nw == 2 || nw == 4))
         (void)persistent_per_cu(variant, B, nw);
 }
 
@@ -978,6 +980,7 @@ void warm_scan_mfma(int variant, int B, int nw) {
 hipError_t launch_scan_mfma(int variant, const ScanMfmaArgs &a, int B, int nw, hipStream_t st) {
     if (a.n_items == 0) return hipSuccess;
     if (nw != 1 && nw != 2 && nw != 4) return hipErrorInvalidValue;
+    if ((uint32_t)B > a.list_stride) return hipErrorInvalidValue;
     uint64_t grid = a.n_items;
     if (a.queue) {
         int dev = 0, cus = 0;
