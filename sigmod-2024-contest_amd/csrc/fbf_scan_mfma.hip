@@ -9,7 +9,7 @@
 // Candidates that pass go to the query's list; the merge kernel then reranks
 // the k' best per query with the exact FP32 distance.
 //
-// Variants (one template, three row encodings):
+// Variants (one template, four row encodings):
 //   BF16: rows as bf16, |x|^2 folded into two spare K slots (hi + lo bf16)
 //         with 1.0 in the matching query slots: score = acc.
 //   FP8:  rows e4m3 with a power-of-two scale per row, queries e4m3 with a
