@@ -36,7 +36,7 @@ def parse():
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--config", type=int, default=1)
-    p.add_argument("--mode", default="bf16", choices=["exact", "sq8", "bf16", "fp8", "sq8sym"])
+    p.add_argument("--mode", default="bf16", choices=["exact", "sq8", "bf16", "fp8", "sq8sym", "exact_bf16"])
     p.add_argument("--kprime", type=int, default=0)
     p.add_argument("--k", type=int, default=100)
     p.add_argument("--n", type=int, default=0, help="override base rows (smaller runs)")
@@ -106,7 +106,7 @@ def main() -> int:
     kw = dict(k=a.k, mode=mode, kprime=a.kprime, chunk_rows=a.chunk_rows, group_queries=a.group_queries,
               persistent=a.persistent, out=out, prepass_div=a.prepass_div, prepass_margin=a.prepass_margin)
     tuned = None
-    if a.autotune and mode != "exact" and not (a.kprime or a.chunk_rows or a.group_queries):
+    if a.autotune and mode not in ("exact", "exact_bf16") and not (a.kprime or a.chunk_rows or a.group_queries):
         t0 = time.time()
         res = pkg.autotune.tune(ds, qd, metas, k=a.k, mode=mode, log=log)
         tuned = dict(res.opts, sample_recall=res.recall)
@@ -149,6 +149,7 @@ def main() -> int:
     elapsed = t_end - t_start
     stats = ds.last_stats()
     rescans = ds.last_rescans()
+    fallbacks = ds.last_fallbacks()
     n_scan, ms_scan = ds.profile_collect("scan")
     n_merge, ms_merge = ds.profile_collect("merge")
     n_rr, ms_rr = ds.profile_collect("rerank")
@@ -161,7 +162,8 @@ def main() -> int:
     log(f"{a.steps} steps in {elapsed:.3f}s: {qps:.0f} QPS; scan {ms_scan / max(1, a.steps):.1f} ms/step "
         f"({n_scan} launches), merge+rerank {ms_merge / max(1, a.steps):.1f} ms/step, "
         f"{stats['work_items']} work items, {stats['pairs_computed'] / max(1, stats['pairs']):.3f}x pairs computed, "
-        f"{rescans} queries rescanned, host planning {ms_plan / max(1, a.steps):.1f} ms/step")
+        f"{rescans} queries rescanned, host planning {ms_plan / max(1, a.steps):.1f} ms/step"
+        + (f", {fallbacks} queries left to the FP32 scan by the certificate" if mode == "exact_bf16" else ""))
 
     ids = out[0].cpu().numpy().view(np.uint32)
     dists = out[1].cpu().numpy()
@@ -169,7 +171,7 @@ def main() -> int:
     # The same batch captured once as a HIP graph (planning and upload
     # untimed), then replayed: the device work alone, with no host planning.
     graph_qps = None
-    if a.graph or (a.graph is None and mode != "exact"):
+    if mode != "exact_bf16" and (a.graph or (a.graph is None and mode != "exact")):
         kwg = {key: kw[key] for key in ("k", "mode", "kprime", "chunk_rows", "group_queries", "persistent",
                                        "prepass_div", "prepass_margin")}
         g = pkg.fbf.SearchGraph(ds, qd, metas, **kwg)
@@ -220,7 +222,8 @@ def main() -> int:
                 "frac": achieved / PEAK_TFLOPS["fp32"], "traffic": committed_traffic(cfg.name, mode, a, kw)}
     else:
         flops = 2.0 * 100 * pairs_per_launch
-        pk = PEAK_TFLOPS["bf16"] if mode in ("bf16", "sq8") else PEAK_TFLOPS["fp8"] if mode == "fp8" else \
+        # exact_bf16: BF16 flops over all its scan launches (the FP32 fallback included).
+        pk = PEAK_TFLOPS["bf16"] if mode in ("bf16", "sq8", "exact_bf16") else PEAK_TFLOPS["fp8"] if mode == "fp8" else \
             PEAK_TFLOPS["int8"]
         achieved = flops / (avg_launch_ms * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": achieved, "peak": pk,
@@ -244,12 +247,14 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"exact": "f32", "sq8": "u8", "bf16": "bf16", "fp8": "fp8e4m3", "sq8sym": "i8"}[mode],
+        "dtype": {"exact": "f32", "sq8": "u8", "bf16": "bf16", "fp8": "fp8e4m3", "sq8sym": "i8",
+                  "exact_bf16": "bf16+f32"}[mode],
         "data": "synthetic (seeded Gaussian mixture, Zipf categories, uniform timestamps)",
         "config": {"workload": cfg.name, "n": n, "d": 100, "nq": nq, "k": a.k, "mode": mode,
                    "kprime": kw["kprime"], "group_queries": kw["group_queries"], "chunk_rows": kw["chunk_rows"],
                    "autotuned": tuned, "per_batch_searches": per_batch, "graph_replay_qps": graph_qps,
-                   "pairs_per_step": pairs, "work_items": stats["work_items"]},
+                   "pairs_per_step": pairs, "work_items": stats["work_items"],
+                   "certificate_fallbacks": fallbacks if mode == "exact_bf16" else None},
         "recall_at_k_vs_fp64": rec,
         "recall_sample": int(sample.size),
         "max_rel_err_vs_fp64": kth_err,

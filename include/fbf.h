@@ -48,7 +48,14 @@ typedef enum fbf_mode {
     FBF_MODE_SQ8 = 1,       /* per-dim uint8 scan (SQ8Quantizer semantics), FP32 rerank of k' */
     FBF_MODE_BF16 = 2,      /* BF16 MFMA q.x + FP32 norms, FP32 rerank of k' */
     FBF_MODE_FP8 = 3,       /* FP8 e4m3 (per-vector scale) MFMA q.x + FP32 norms, FP32 rerank */
-    FBF_MODE_SQ8SYM = 4     /* symmetric int8 scan (router's bruteforce()), FP32 rerank of k' */
+    FBF_MODE_SQ8SYM = 4,    /* symmetric int8 scan (router's bruteforce()), FP32 rerank of k' */
+    FBF_MODE_EXACT_BF16 = 5 /* the FBF_MODE_EXACT_F32 results (same ids and distances, Bruteforce
+                               utils.h:80-158) from the BF16 MFMA scan and an FP32 rerank of k'
+                               (default 896; rows per work item default 262144) candidates: a
+                               query is certified when the k'-th BF16
+                               score, less a bound on its rounding error, exceeds the k-th exact
+                               distance; the others are searched again by the FP32 scan.  Needs
+                               FBF_REPR_BF16 | FBF_REPR_F32; waits for the device (no capture). */
 } fbf_mode;
 
 /* Representations built by fbf_dataset_create (bit flags). */
@@ -202,6 +209,12 @@ fbf_status fbf_last_search_stats(const fbf_dataset *ds, uint64_t *pairs, uint64_
  * (0 when the search ran no prefix pass).  Waits for that search's verify
  * step to finish. */
 fbf_status fbf_last_search_rescans(fbf_dataset *ds, uint64_t *queries);
+
+/* Number of queries of the last fbf_search on this dataset that
+ * FBF_MODE_EXACT_BF16 could not certify and searched again with the FP32
+ * scan (0 after searches in other modes).  No reference counterpart:
+ * measurement of the certified mode. */
+fbf_status fbf_last_search_fallbacks(const fbf_dataset *ds, uint64_t *queries);
 
 /* Text of the last error on the calling thread ("" when none). */
 const char *fbf_last_error(void);

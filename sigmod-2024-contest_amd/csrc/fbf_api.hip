@@ -112,6 +112,8 @@ struct ProfEvent {
 struct Scratch {
     DevBuf d_items, d_iq, d_slot_off, d_lists, d_counts, d_queue, d_q, d_out_ids, d_out_d, d_gtau,
         d_tau0, d_flags, d_pre_end;
+    // FBF_MODE_EXACT_BF16: per-query certificates and the uncertified subset.
+    DevBuf d_cert, d_sub_idx, d_sub_q, d_sub_ids, d_sub_d;
     HostPinned h_stage[2], h_nflag;
     hipEvent_t ev_stage[2] = {nullptr, nullptr};  // recorded after the copies out of h_stage[i]
     int stage_next = 0;
@@ -119,8 +121,9 @@ struct Scratch {
     hipStream_t last_stream = nullptr;
     bool used = false;
     void release() {
-        for (DevBuf *b : {&d_items, &d_iq, &d_slot_off, &d_lists, &d_counts, &d_queue,
-                          &d_q, &d_out_ids, &d_out_d, &d_gtau, &d_tau0, &d_flags, &d_pre_end})
+        for (DevBuf *b : {&d_items, &d_iq, &d_slot_off, &d_lists, &d_counts, &d_queue, &d_q, &d_out_ids,
+                          &d_out_d, &d_gtau, &d_tau0, &d_flags, &d_pre_end, &d_cert, &d_sub_idx, &d_sub_q,
+                          &d_sub_ids, &d_sub_d})
             b->release();
         for (int i = 0; i < 2; ++i) {
             h_stage[i].release();
@@ -146,14 +149,17 @@ struct fbf_dataset {
     LayoutDev lay[2];
     float *sq8_mi = nullptr, *sq8_dif = nullptr, *sq8_w = nullptr;  // [d] each
     float alpha = 0.f;           // symmetric int8 quantiser scale
+    float max_norm = 0.f;        // largest row norm |x| (rounded up; BF16 builds), for FBF_MODE_EXACT_BF16
     std::vector<float> h_mi, h_dif;
     Scratch scr;
     std::mutex mu;
+    std::mutex mu_cert;          // one FBF_MODE_EXACT_BF16 search at a time (its subset buffers)
     // measurement
     bool profile = false;
     std::vector<ProfEvent> events;
     uint64_t last_pairs = 0, last_pairs_computed = 0, last_items = 0;
     bool last_seeded = false;  // the last search ran the seeded prefix pass (h_nflag is live)
+    uint64_t last_fallbacks = 0;  // FBF_MODE_EXACT_BF16: queries searched again by the FP32 scan
 };
 
 // ---------------------------------------------------------------- helpers
@@ -262,6 +268,16 @@ static fbf_status dataset_build(fbf_dataset *ds, const float *labels, const floa
     if (repr & (FBF_REPR_BF16 | FBF_REPR_FP8)) {
         HIP_TRY(norms.reserve(n * sizeof(float)));
         HIP_TRY(launch_row_norms(ds->base_rows, n, norms.as<float>(), st));
+        // The largest |x|^2 (its bits order like the value: norms are >= 0).
+        HIP_TRY(absbits.reserve(sizeof(uint32_t)));
+        HIP_TRY(hipMemsetAsync(absbits.p, 0, sizeof(uint32_t), st));
+        HIP_TRY(launch_absmax(norms.as<float>(), n, absbits.as<uint32_t>(), st));
+        uint32_t bits = 0;
+        HIP_TRY(hipMemcpyAsync(&bits, absbits.p, 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        float m2 = 0.f;
+        std::memcpy(&m2, &bits, 4);
+        ds->max_norm = (float)(std::sqrt((double)m2) * (1.0 + 1e-5));
     }
     if (repr & FBF_REPR_SQ8) {
         HIP_TRY(hipMalloc(&ds->sq8_mi, kDim * sizeof(float)));
@@ -680,9 +696,11 @@ static void xcd_runs(const std::vector<WorkItem> &items, uint64_t i0, uint64_t i
 }
 
 static fbf_status mode_check(const fbf_dataset *ds, uint32_t mode) {
-    static const uint32_t need[5] = {FBF_REPR_F32, FBF_REPR_SQ8, FBF_REPR_BF16, FBF_REPR_FP8, FBF_REPR_SQ8SYM};
-    if (mode > FBF_MODE_SQ8SYM) return fail(FBF_ERR_INVALID_ARG, "unknown mode");
-    if (!(ds->repr & need[mode])) return fail(FBF_ERR_NO_REPR, "dataset was created without this mode's representation");
+    static const uint32_t need[6] = {FBF_REPR_F32, FBF_REPR_SQ8,    FBF_REPR_BF16,
+                                     FBF_REPR_FP8, FBF_REPR_SQ8SYM, FBF_REPR_BF16 | FBF_REPR_F32};
+    if (mode > FBF_MODE_EXACT_BF16) return fail(FBF_ERR_INVALID_ARG, "unknown mode");
+    if ((ds->repr & need[mode]) != need[mode])
+        return fail(FBF_ERR_NO_REPR, "dataset was created without this mode's representation");
     return FBF_OK;
 }
 
@@ -697,7 +715,8 @@ struct GraphOut {
 // work (everything after the plan upload) into a graph instead of running it.
 static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, const float *queries_dev,
                               const float *metas, uint64_t nq, uint32_t k, const fbf_search_opts *opts,
-                              uint32_t *out_ids_dev, float *out_dists_dev, hipStream_t st, GraphOut *cap) {
+                              uint32_t *out_ids_dev, float *out_dists_dev, hipStream_t st, GraphOut *cap,
+                              uint32_t *cert = nullptr) {
     if (!ds || !metas || (nq && (!queries_dev || !out_ids_dev || !out_dists_dev)))
         return fail(FBF_ERR_INVALID_ARG, "null argument");
     if (k == 0 || k > 512) return fail(FBF_ERR_INVALID_ARG, "k must be in [1, 512]");
@@ -706,6 +725,8 @@ static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, 
     if (opts) o = *opts;
     fbf_status s = mode_check(ds, o.mode);
     if (s != FBF_OK) return s;
+    if (o.mode == FBF_MODE_EXACT_BF16)
+        return fail(FBF_ERR_INVALID_ARG, "FBF_MODE_EXACT_BF16 waits for the device between its steps; it cannot be captured");
     const bool exact = o.mode == FBF_MODE_EXACT_F32;
     const bool mfma = o.mode == FBF_MODE_BF16 || o.mode == FBF_MODE_FP8 || o.mode == FBF_MODE_SQ8 ||
                       o.mode == FBF_MODE_SQ8SYM;
@@ -871,6 +892,8 @@ static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, 
         m.key_pos = mfma ? 1 : 0;  // the MFMA scans key candidates by layout position
         m.ids_lay[0] = ds->lay[0].ids;
         m.ids_lay[1] = ds->lay[1].ids;
+        m.cert = cert;
+        m.cert_norm = ds->max_norm;
         return m;
     };
     auto run_merge = [&](MergeArgs &m) -> fbf_status {
@@ -1026,10 +1049,88 @@ static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, 
     return sw;
 }
 
+// FBF_MODE_EXACT_BF16: the BF16 search with a certificate per query, then the
+// FP32 scan over the uncertified queries only, whose rows replace theirs.
+// Certified rows equal the exact search's (same candidates, canonical FP32
+// distances, (distance, id) order), so the whole output does.
+static fbf_status search_exact_bf16(fbf_dataset *ds, const float *queries_dev, const float *metas, uint64_t nq,
+                                    uint32_t k, const fbf_search_opts &o, uint32_t *out_ids_dev,
+                                    float *out_dists_dev, hipStream_t st) {
+    fbf_status s = mode_check(ds, FBF_MODE_EXACT_BF16);
+    if (s != FBF_OK) return s;
+    if (!metas || (nq && (!queries_dev || !out_ids_dev || !out_dists_dev)))
+        return fail(FBF_ERR_INVALID_ARG, "null argument");
+    if (k == 0 || k > 512) return fail(FBF_ERR_INVALID_ARG, "k must be in [1, 512]");
+    std::lock_guard<std::mutex> lk(ds->mu_cert);
+    if (nq == 0) return FBF_OK;
+    HIP_TRY(hipSetDevice(ds->device));
+    fbf_search_opts ob = o;
+    ob.mode = FBF_MODE_BF16;
+    if (ob.kprime == 0) ob.kprime = 896;
+    // Lists of up to 1024 keys: 256 Ki rows per work item by default keeps
+    // their memory (slots x 8 KiB) near 12 GB for 100 k queries over 10 M rows.
+    if (ob.chunk_rows == 0) ob.chunk_rows = 262144;
+    Scratch &S = ds->scr;
+    HIP_TRY(S.d_cert.reserve(nq * sizeof(uint32_t)));
+    s = search_impl(ds, S, true, queries_dev, metas, nq, k, &ob, out_ids_dev, out_dists_dev, st, nullptr,
+                    S.d_cert.as<uint32_t>());
+    if (s != FBF_OK) return s;
+    // The search statistics stay those of the BF16 search (the fallback is
+    // counted by last_fallbacks).
+    uint64_t pairs = 0, pairs_c = 0, items = 0;
+    bool seeded = false;
+    {
+        std::lock_guard<std::mutex> lock(ds->mu);
+        pairs = ds->last_pairs;
+        pairs_c = ds->last_pairs_computed;
+        items = ds->last_items;
+        seeded = ds->last_seeded;
+    }
+    std::vector<uint32_t> hc(nq);
+    HIP_TRY(hipMemcpyAsync(hc.data(), S.d_cert.p, nq * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    std::vector<uint32_t> sub;
+    for (uint64_t i = 0; i < nq; ++i)
+        if (hc[i] == 0) sub.push_back((uint32_t)i);
+    if (!sub.empty()) {
+        const uint64_t m = sub.size();
+        HIP_TRY(S.d_sub_idx.reserve(m * sizeof(uint32_t)));
+        HIP_TRY(S.d_sub_q.reserve(m * kDim * sizeof(float)));
+        HIP_TRY(S.d_sub_ids.reserve(m * k * sizeof(uint32_t)));
+        HIP_TRY(S.d_sub_d.reserve(m * k * sizeof(float)));
+        std::vector<float> msub(m * 4);
+        for (uint64_t j = 0; j < m; ++j) std::memcpy(&msub[4 * j], metas + 4 * (uint64_t)sub[j], 4 * sizeof(float));
+        HIP_TRY(hipMemcpyAsync(S.d_sub_idx.p, sub.data(), m * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        HIP_TRY(launch_gather_rows(queries_dev, S.d_sub_idx.as<uint32_t>(), m, S.d_sub_q.as<float>(), st));
+        fbf_search_opts oe{};
+        oe.mode = FBF_MODE_EXACT_F32;
+        s = search_impl(ds, S, true, S.d_sub_q.as<float>(), msub.data(), m, k, &oe, S.d_sub_ids.as<uint32_t>(),
+                        S.d_sub_d.as<float>(), st, nullptr);
+        if (s != FBF_OK) return s;
+        HIP_TRY(launch_scatter_results(S.d_sub_ids.as<uint32_t>(), S.d_sub_d.as<float>(), S.d_sub_idx.as<uint32_t>(),
+                                       m, k, out_ids_dev, out_dists_dev, st));
+        // The subset buffers are reused by the next call: let this one finish.
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+    std::lock_guard<std::mutex> lock(ds->mu);
+    ds->last_fallbacks = sub.size();
+    ds->last_pairs = pairs;
+    ds->last_pairs_computed = pairs_c;
+    ds->last_items = items;
+    ds->last_seeded = seeded;
+    return FBF_OK;
+}
+
 extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, const float *metas,
                                  uint64_t nq, uint32_t k, const fbf_search_opts *opts,
                                  uint32_t *out_ids_dev, float *out_dists_dev, void *stream) {
     if (!ds) return fail(FBF_ERR_INVALID_ARG, "null argument");
+    if (opts && opts->mode == FBF_MODE_EXACT_BF16)
+        return search_exact_bf16(ds, queries_dev, metas, nq, k, *opts, out_ids_dev, out_dists_dev, (hipStream_t)stream);
+    {
+        std::lock_guard<std::mutex> lock(ds->mu);
+        ds->last_fallbacks = 0;
+    }
     return search_impl(ds, ds->scr, true, queries_dev, metas, nq, k, opts, out_ids_dev, out_dists_dev,
                        (hipStream_t)stream, nullptr);
 }
@@ -1194,6 +1295,12 @@ extern "C" fbf_status fbf_last_search_rescans(fbf_dataset *ds, uint64_t *queries
     if (!ds->last_seeded) return FBF_OK;
     HIP_TRY(hipEventSynchronize(ds->scr.ev_nflag));
     *queries = *static_cast<const uint32_t *>(ds->scr.h_nflag.p);
+    return FBF_OK;
+}
+
+extern "C" fbf_status fbf_last_search_fallbacks(const fbf_dataset *ds, uint64_t *queries) {
+    if (!ds || !queries) return fail(FBF_ERR_INVALID_ARG, "null argument");
+    *queries = ds->last_fallbacks;
     return FBF_OK;
 }
 

@@ -152,6 +152,13 @@ struct MergeArgs {
     int key_int;               // keys carry an integer distance (symmetric int8 codes)
     int key_pos;               // key low word = (layout << 31) | layout position, not the id
     const uint32_t *ids_lay[2];// position -> id per layout (key_pos)
+    // Certificate of the BF16 scan's candidates (rerank only; null: none).
+    // cert[q] = 1 when no row outside the k' candidates can have a canonical
+    // FP32 distance at or below the k-th reranked one: the k'-th BF16 score
+    // less its rounding-error bound (from |q| and cert_norm = the largest row
+    // norm), plus |q|^2, lies above it.
+    uint32_t *cert;
+    float cert_norm;
     // Seeded scans.  pass 0: plain merge.  pass 3 (after the prefix pass):
     // merge only the slots [slot_off[q], pre_end[q]) to their kprime-th key and
     // store it as the seed in tau0[q] and gtau[q]; nothing is emitted.  pass 1:
@@ -169,6 +176,11 @@ struct MergeArgs {
 hipError_t launch_scan_f32(const ScanArgs &a, int B, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st);
 hipError_t launch_merge(const MergeArgs &a, hipStream_t st);
+// dst[i][:] = src[idx[i]][:] (kDim floats per row), i < n.
+hipError_t launch_gather_rows(const float *src, const uint32_t *idx, uint64_t n, float *dst, hipStream_t st);
+// out_ids[idx[i]][:] = ids[i][:], out_d[idx[i]][:] = d[i][:] (k per row), i < n.
+hipError_t launch_scatter_results(const uint32_t *ids, const float *d, const uint32_t *idx, uint64_t n, uint32_t k,
+                                  uint32_t *out_ids, float *out_d, hipStream_t st);
 hipError_t launch_scan_sq8sym(const ScanSymArgs &a, int B, hipStream_t st);
 // variant: 0 BF16, 1 FP8, 2 SQ8, 3 symmetric int8; B: list capacity (128,
 // 256, 512, 1024; at most a.list_stride); nw: waves per workgroup (1, 2, 4),

@@ -275,6 +275,29 @@ __global__ __launch_bounds__(64) void merge_kernel(MergeArgs a) {
         }
         __syncthreads();
         wave_sort_lds<B>(list);
+        if (a.cert) {
+            // Rows outside the candidates have BF16 scores at or above the
+            // k'-th one, ts (keys are unique, so "outside" means "key above
+            // tau").  A BF16 score |x|^2_hi + |x|^2_lo - 2 bf16(q).bf16(x)
+            // (FP32 accumulation of 102 exact products) differs from
+            // |x|^2 - 2 q.x by at most 2 |q| |x| (2^-8 + 2^-18 + 112 * 2^-24)
+            // + |x|^2 (2^-18 + 214 * 2^-24) <= |q| |x| 0.00784 + |x|^2 1.7e-5;
+            // the canonical FP32 distances and |q|^2 carry relative errors
+            // below 1e-5.  Fewer than k' candidates (tau = kKeyMax) means the
+            // range held no more rows.
+            uint32_t ok = 1u;
+            if (tau != kKeyMax && cnt >= a.k) {
+                float qn = 0.f;
+                for (int d = 0; d < kDim; ++d) qn = __builtin_fmaf(qv[d], qv[d], qn);
+                const float xm = a.cert_norm;
+                const float eps = 1.01f * (__builtin_sqrtf(qn) * xm * 0.00784f + xm * xm * 1.7e-5f);
+                const float ts = ord2f((uint32_t)(tau >> 32));
+                const float lower = (ts + qn * (1.0f - 2e-5f) - eps) * (1.0f - 3e-5f);
+                const float dk = ord2f((uint32_t)(list[a.k - 1] >> 32));
+                ok = lower > dk ? 1u : 0u;
+            }
+            if (lane == 0) a.cert[q] = ok;
+        }
         if (cnt > a.k) cnt = a.k;
     }
     for (uint32_t i = lane; i < a.k; i += 64) {
@@ -290,6 +313,41 @@ __global__ __launch_bounds__(64) void merge_kernel(MergeArgs a) {
         a.out_ids[q * a.k + i] = id;
         a.out_dists[q * a.k + i] = d;
     }
+}
+
+__global__ void gather_rows_kernel(const float *__restrict__ src, const uint32_t *__restrict__ idx, uint64_t n,
+                                   float *__restrict__ dst) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * kDim) return;
+    const uint64_t i = t / kDim, d = t % kDim;
+    dst[t] = src[(uint64_t)idx[i] * kDim + d];
+}
+
+__global__ void scatter_results_kernel(const uint32_t *__restrict__ ids, const float *__restrict__ dd,
+                                       const uint32_t *__restrict__ idx, uint64_t n, uint32_t k,
+                                       uint32_t *__restrict__ out_ids, float *__restrict__ out_d) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * k) return;
+    const uint64_t i = t / k, j = t % k;
+    const uint64_t o = (uint64_t)idx[i] * k + j;
+    out_ids[o] = ids[t];
+    out_d[o] = dd[t];
+}
+
+hipError_t launch_gather_rows(const float *src, const uint32_t *idx, uint64_t n, float *dst, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const uint64_t total = n * kDim;
+    hipLaunchKernelGGL(gather_rows_kernel, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, st, src, idx, n, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter_results(const uint32_t *ids, const float *d, const uint32_t *idx, uint64_t n, uint32_t k,
+                                  uint32_t *out_ids, float *out_d, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const uint64_t total = n * k;
+    hipLaunchKernelGGL(scatter_results_kernel, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, st, ids, d, idx,
+                       n, k, out_ids, out_d);
+    return hipGetLastError();
 }
 
 hipError_t launch_merge(const MergeArgs &a, hipStream_t st) {

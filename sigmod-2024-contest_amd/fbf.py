@@ -24,13 +24,13 @@ LIB_PATH = os.path.join(HERE, "lib", "libfbf.so")
 PAD_ID = 0xFFFFFFFF
 CODE_BYTES = 112
 
-MODE_EXACT_F32, MODE_SQ8, MODE_BF16, MODE_FP8, MODE_SQ8SYM = range(5)
+MODE_EXACT_F32, MODE_SQ8, MODE_BF16, MODE_FP8, MODE_SQ8SYM, MODE_EXACT_BF16 = range(6)
 MODES = {"exact": MODE_EXACT_F32, "sq8": MODE_SQ8, "bf16": MODE_BF16, "fp8": MODE_FP8,
-         "sq8sym": MODE_SQ8SYM}
+         "sq8sym": MODE_SQ8SYM, "exact_bf16": MODE_EXACT_BF16}
 REPR_F32, REPR_SQ8, REPR_BF16, REPR_FP8, REPR_SQ8SYM = 0x1, 0x2, 0x4, 0x8, 0x10
 REPR_ALL = 0x1F
 MODE_REPR = {MODE_EXACT_F32: REPR_F32, MODE_SQ8: REPR_SQ8, MODE_BF16: REPR_BF16,
-             MODE_FP8: REPR_FP8, MODE_SQ8SYM: REPR_SQ8SYM}
+             MODE_FP8: REPR_FP8, MODE_SQ8SYM: REPR_SQ8SYM, MODE_EXACT_BF16: REPR_BF16 | REPR_F32}
 
 # Every symbol include/fbf.h declares (checked by the CPU test suite).
 EXPORTS = [
@@ -39,7 +39,7 @@ EXPORTS = [
     "fbf_unregister_codes", "fbf_bruteforce", "fbf_bruteforce_subgraph", "fbf_bruteforce_batch",
     "fbf_profile_enable", "fbf_profile_collect", "fbf_last_search_stats", "fbf_last_search_rescans",
     "fbf_last_error", "fbf_version", "fbf_search_graph_create", "fbf_search_graph_launch",
-    "fbf_search_graph_destroy",
+    "fbf_search_graph_destroy", "fbf_last_search_fallbacks",
 ]
 
 
@@ -86,6 +86,7 @@ def lib() -> C.CDLL:
     L.fbf_last_search_stats.argtypes = [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                         C.POINTER(C.c_uint64)]
     L.fbf_last_search_rescans.argtypes = [_P, C.POINTER(C.c_uint64)]
+    L.fbf_last_search_fallbacks.argtypes = [_P, C.POINTER(C.c_uint64)]
     L.fbf_search_graph_create.argtypes = [_P, _P, _P, C.c_uint64, C.c_uint32, C.POINTER(SearchOpts), _P, _P,
                                           C.POINTER(_P)]
     L.fbf_search_graph_launch.argtypes = [_P, _P]
@@ -233,6 +234,12 @@ class Dataset:
         """Queries of the last search rescanned after their seed failed."""
         n = C.c_uint64(0)
         _check(lib().fbf_last_search_rescans(self.h, C.byref(n)), "fbf_last_search_rescans")
+        return n.value
+
+    def last_fallbacks(self) -> int:
+        """Queries of the last exact_bf16 search that the certificate left to the FP32 scan."""
+        n = C.c_uint64(0)
+        _check(lib().fbf_last_search_fallbacks(self.h, C.byref(n)), "fbf_last_search_fallbacks")
         return n.value
 
 
