@@ -51,7 +51,7 @@ typedef enum fbf_mode {
     FBF_MODE_SQ8SYM = 4,    /* symmetric int8 scan (router's bruteforce()), FP32 rerank of k' */
     FBF_MODE_EXACT_BF16 = 5 /* the FBF_MODE_EXACT_F32 results (same ids and distances, Bruteforce
                                utils.h:80-158) from the BF16 MFMA scan and an FP32 rerank of k'
-                               (default 896; rows per work item default 262144) candidates: a
+                               (default 640; rows per work item default 262144) candidates: a
                                query is certified when the k'-th BF16
                                score, less a bound on its rounding error, exceeds the k-th exact
                                distance; the others are searched again by the FP32 scan.  Needs
