@@ -75,7 +75,7 @@ typedef struct fbf_dataset fbf_dataset;
  * <= 256 for the MFMA scans, default 128). */
 typedef struct fbf_search_opts {
     uint32_t mode;        /* fbf_mode */
-    uint32_t kprime;      /* 0 -> 3k/2 (BF16, SQ8) or 4k (FP8, SQ8SYM), at most 896 */
+    uint32_t kprime;      /* 0 -> 3k/2 (BF16, SQ8), 4k (FP8, SQ8SYM) or 640 (EXACT_BF16), at most 896 */
     uint32_t chunk_rows;  /* 0 -> default */
     uint32_t group_queries;
     uint32_t persistent;  /* scan launch form: 0 -> default (MFMA modes: persistent; others: one
