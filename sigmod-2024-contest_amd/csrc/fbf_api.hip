@@ -838,10 +838,11 @@ static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, 
     // is large (a scan whose threshold is still open then compacts rarely);
     // each of the two halves a lane appends to keeps room above ceil(K / 2) + 16.
     const int B = mfma ? (K <= 160 ? 256 : K <= 300 ? 512 : 1024) : list_capacity((int)K);
-    // The prefix pass keeps only kpre keys: with a small kpre its lists use
-    // 128 of their B slots, so each compaction sorts half as many keys and
-    // its threshold tightens after fewer appends.
-    const int B_pre = (seeded && kpre <= 64 && B >= 128) ? 128 : B;
+    // The prefix pass keeps only kpre keys: its lists use the capacity the
+    // rule above gives for kpre (128 below 64 keys), at most their B slots,
+    // so each compaction sorts fewer keys and the threshold tightens after
+    // fewer appends.
+    const int B_pre = !seeded ? B : std::min(B, kpre <= 64 ? 128 : kpre <= 160 ? 256 : kpre <= 300 ? 512 : 1024);
     s = upload_plan(S, P, nq, B, st);
     if (s != FBF_OK) return s;
     if (prof) {
