@@ -232,18 +232,31 @@ static fbf_status dataset_build(fbf_dataset *ds, const float *labels, const floa
     std::vector<int32_t> lab(n);
     for (uint64_t i = 0; i < n; ++i) lab[i] = label_int(labels[i]);
     std::vector<uint32_t> pb(n), pt(n);
-    std::iota(pb.begin(), pb.end(), 0u);
-    std::iota(pt.begin(), pt.end(), 0u);
-    // SortDataset comparators (hybrid_graph.cpp:14-27) with id as the final key.
-    std::sort(pb.begin(), pb.end(), [&](uint32_t a, uint32_t b) {
-        if (lab[a] != lab[b]) return lab[a] < lab[b];
-        if (timestamps[a] != timestamps[b]) return timestamps[a] < timestamps[b];
-        return a < b;
-    });
-    std::sort(pt.begin(), pt.end(), [&](uint32_t a, uint32_t b) {
-        if (timestamps[a] != timestamps[b]) return timestamps[a] < timestamps[b];
-        return a < b;
-    });
+    {
+        // SortDataset's orders (comparators hybrid_graph.cpp:14-27) with the id
+        // as the final key, as stable radix sorts of packed keys on the device:
+        // (label ^ 2^31) << 32 | f2ord(timestamp), and f2ord(timestamp).  -0.0
+        // and +0.0 compare equal as floats, so both get the key of +0.0.
+        std::vector<uint64_t> kb(n), kt(n);
+        for (uint64_t i = 0; i < n; ++i) {
+            const float t = timestamps[i] == 0.0f ? 0.0f : timestamps[i];
+            const uint32_t o = f2ord(t);
+            kb[i] = ((uint64_t)((uint32_t)lab[i] ^ 0x80000000u) << 32) | o;
+            kt[i] = o;
+        }
+        struct Guard {
+            DevBuf k, p;
+            ~Guard() { k.release(); p.release(); }
+        } g;
+        HIP_TRY(g.k.reserve(n * sizeof(uint64_t)));
+        HIP_TRY(g.p.reserve(n * sizeof(uint32_t)));
+        HIP_TRY(hipMemcpy(g.k.p, kb.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice));
+        HIP_TRY(sort_rows_by_key(g.k.as<uint64_t>(), n, 64, g.p.as<uint32_t>(), st));
+        HIP_TRY(hipMemcpy(pb.data(), g.p.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(g.k.p, kt.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice));
+        HIP_TRY(sort_rows_by_key(g.k.as<uint64_t>(), n, 32, g.p.as<uint32_t>(), st));
+        HIP_TRY(hipMemcpy(pt.data(), g.p.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    }
     ds->lab_b.resize(n);
     ds->ts_b.resize(n);
     ds->ts_t.resize(n);

@@ -176,6 +176,9 @@ struct MergeArgs {
 hipError_t launch_scan_f32(const ScanArgs &a, int B, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st);
 hipError_t launch_merge(const MergeArgs &a, hipStream_t st);
+// Row ids 0..n-1 stably sorted by keys_dev[0..n) (bits [0, end_bit), end_bit
+// 32 or 64) into ids_out_dev; synchronises st.
+hipError_t sort_rows_by_key(const uint64_t *keys_dev, uint64_t n, int end_bit, uint32_t *ids_out_dev, hipStream_t st);
 // dst[i][:] = src[idx[i]][:] (kDim floats per row), i < n.
 hipError_t launch_gather_rows(const float *src, const uint32_t *idx, uint64_t n, float *dst, hipStream_t st);
 // out_ids[idx[i]][:] = ids[i][:], out_d[idx[i]][:] = d[i][:] (k per row), i < n.
