@@ -48,6 +48,8 @@ def parse():
     p.add_argument("--prepass-margin", type=int, default=0)
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--exact-bf16", action=argparse.BooleanOptionalAction, default=True,
+                   help="in BF16 runs, also time the certified exact mode (results identical to exact FP32)")
     p.add_argument("--recall-sample", type=int, default=16, help="queries checked against FP64")
     p.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True,
                    help="pick group_queries / chunk_rows / kprime with the project's autotuner (untimed) "
@@ -190,6 +192,29 @@ def main() -> int:
         log(f"graph replays: {a.steps} in {dt:.3f}s: {graph_qps:.0f} QPS ({1e3 * dt / a.steps:.1f} ms/step), "
             f"ids equal to the eager search: {same}")
 
+    # The certified exact mode on the same dataset and batch (BF16 scan + FP32
+    # rerank + certificate + FP32 fallback): exact results, timed like the
+    # main line (1 warmup, then min(steps, 3) timed searches).
+    exact_cert = None
+    if mode == "bf16" and a.exact_bf16:
+        kwe = dict(k=a.k, mode="exact_bf16", out=out)
+        ds.search(qd, metas, **kwe)
+        torch.cuda.synchronize()
+        ne = min(a.steps, 3)
+        t0 = time.perf_counter()
+        for i in range(ne):
+            ds.search(qd, metas, **kwe)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        exact_cert = {"qps": nq * ne / dt, "ms_per_step": 1e3 * dt / ne, "steps": ne,
+                      "fallbacks": ds.last_fallbacks()}
+        log(f"certified exact (exact_bf16): {ne} steps in {dt:.3f}s: {nq * ne / dt:.0f} QPS, "
+            f"{exact_cert['fallbacks']} queries left to the FP32 scan")
+        ds.search(qd, metas, **kw)  # the main line's results back in out for the recall check
+        torch.cuda.synchronize()
+        ids = out[0].cpu().numpy().view(np.uint32)
+        dists = out[1].cpu().numpy()
+
     # Recall@k against FP64 exact on a type-stratified sample (CPU oracle).
     orc = ge.load_oracle()
     rng = np.random.default_rng(0)
@@ -254,7 +279,8 @@ def main() -> int:
                    "kprime": kw["kprime"], "group_queries": kw["group_queries"], "chunk_rows": kw["chunk_rows"],
                    "autotuned": tuned, "per_batch_searches": per_batch, "graph_replay_qps": graph_qps,
                    "pairs_per_step": pairs, "work_items": stats["work_items"],
-                   "certificate_fallbacks": fallbacks if mode == "exact_bf16" else None},
+                   "certificate_fallbacks": fallbacks if mode == "exact_bf16" else None,
+                   "certified_exact": exact_cert},
         "recall_at_k_vs_fp64": rec,
         "recall_sample": int(sample.size),
         "max_rel_err_vs_fp64": kth_err,
