@@ -51,7 +51,7 @@ typedef enum fbf_mode {
     FBF_MODE_SQ8SYM = 4,    /* symmetric int8 scan (router's bruteforce()), FP32 rerank of k' */
     FBF_MODE_EXACT_BF16 = 5 /* the FBF_MODE_EXACT_F32 results (same ids and distances, Bruteforce
                                utils.h:80-158) from the BF16 MFMA scan and an FP32 rerank of k'
-                               (default 640; rows per work item default 262144) candidates: a
+                               (default 720; rows per work item default 262144) candidates: a
                                query is certified when the k'-th BF16
                                score, less a bound on its rounding error, exceeds the k-th exact
                                distance; the others are searched again by the FP32 scan.  Needs
@@ -75,7 +75,7 @@ typedef struct fbf_dataset fbf_dataset;
  * <= 256 for the MFMA scans, default 128). */
 typedef struct fbf_search_opts {
     uint32_t mode;        /* fbf_mode */
-    uint32_t kprime;      /* 0 -> 3k/2 (BF16, SQ8), 4k (FP8, SQ8SYM) or 640 (EXACT_BF16), at most 896 */
+    uint32_t kprime;      /* 0 -> 3k/2 (BF16, SQ8), 4k (FP8, SQ8SYM) or 720 (EXACT_BF16), at most 896 */
     uint32_t chunk_rows;  /* 0 -> default */
     uint32_t group_queries;
     uint32_t persistent;  /* scan launch form: 0 -> default (MFMA modes: persistent; others: one

@@ -150,6 +150,7 @@ struct fbf_dataset {
     float *sq8_mi = nullptr, *sq8_dif = nullptr, *sq8_w = nullptr;  // [d] each
     float alpha = 0.f;           // symmetric int8 quantiser scale
     float max_norm = 0.f;        // largest row norm |x| (rounded up; BF16 builds), for FBF_MODE_EXACT_BF16
+    float bf16_rho = 0.f;        // largest |bf16(x) - x| / |x| over the rows (rounded up), same use
     std::vector<float> h_mi, h_dif;
     Scratch scr;
     std::mutex mu;
@@ -291,6 +292,13 @@ static fbf_status dataset_build(fbf_dataset *ds, const float *labels, const floa
         float m2 = 0.f;
         std::memcpy(&m2, &bits, 4);
         ds->max_norm = (float)(std::sqrt((double)m2) * (1.0 + 1e-5));
+        HIP_TRY(hipMemsetAsync(absbits.p, 0, sizeof(uint32_t), st));
+        HIP_TRY(launch_bf16_err_max(ds->base_rows, n, absbits.as<uint32_t>(), st));
+        HIP_TRY(hipMemcpyAsync(&bits, absbits.p, 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        float rho = 0.f;
+        std::memcpy(&rho, &bits, 4);
+        ds->bf16_rho = (float)((double)rho * 1.001 + 1e-7);
     }
     if (repr & FBF_REPR_SQ8) {
         HIP_TRY(hipMalloc(&ds->sq8_mi, kDim * sizeof(float)));
@@ -908,6 +916,7 @@ static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, 
         m.ids_lay[1] = ds->lay[1].ids;
         m.cert = cert;
         m.cert_norm = ds->max_norm;
+        m.cert_rho = ds->bf16_rho;
         return m;
     };
     auto run_merge = [&](MergeArgs &m) -> fbf_status {
@@ -1080,10 +1089,10 @@ static fbf_status search_exact_bf16(fbf_dataset *ds, const float *queries_dev, c
     HIP_TRY(hipSetDevice(ds->device));
     fbf_search_opts ob = o;
     ob.mode = FBF_MODE_BF16;
-    // k' = 640: on 10 M rows x 100 k queries all but ~0.03% of the queries
+    // k' = 720: on 10 M rows x 100 k queries all but ~0.1% of the queries
     // are certified; a larger k' leaves the 1024-key lists too little room
     // above k' / 2 between compactions, a smaller one certifies fewer.
-    if (ob.kprime == 0) ob.kprime = 640;
+    if (ob.kprime == 0) ob.kprime = 720;
     // Lists of up to 1024 keys: 256 Ki rows per work item by default keeps
     // their memory (slots x 8 KiB) near 12 GB for 100 k queries over 10 M rows.
     if (ob.chunk_rows == 0) ob.chunk_rows = 262144;

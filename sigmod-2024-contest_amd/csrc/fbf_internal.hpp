@@ -51,6 +51,11 @@ __host__ __device__ inline float ord2f(uint32_t o) {
     uint32_t u = (o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o;
     return __builtin_bit_cast(float, u);
 }
+// f rounded to bfloat16 (nearest, ties to even), as a float.
+__host__ __device__ inline float bf16_round(float f) {
+    const uint32_t u = __builtin_bit_cast(uint32_t, f);
+    return __builtin_bit_cast(float, ((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16) << 16);
+}
 
 // Read-only data that every lane of a wave reads at the same address (work
 // items, query vectors) is accessed through the constant address space so the
@@ -155,10 +160,12 @@ struct MergeArgs {
     // Certificate of the BF16 scan's candidates (rerank only; null: none).
     // cert[q] = 1 when no row outside the k' candidates can have a canonical
     // FP32 distance at or below the k-th reranked one: the k'-th BF16 score
-    // less its rounding-error bound (from |q| and cert_norm = the largest row
-    // norm), plus |q|^2, lies above it.
+    // less its rounding-error bound, plus |q|^2, lies above it.  The bound
+    // uses cert_norm (the largest row norm) and cert_rho (the largest
+    // |bf16(x) - x| / |x| over the rows), both rounded up.
     uint32_t *cert;
     float cert_norm;
+    float cert_rho;
     // Seeded scans.  pass 0: plain merge.  pass 3 (after the prefix pass):
     // merge only the slots [slot_off[q], pre_end[q]) to their kprime-th key and
     // store it as the seed in tau0[q] and gtau[q]; nothing is emitted.  pass 1:
@@ -197,6 +204,9 @@ void warm_scan_mfma(int variant, int B, int nw);
 hipError_t launch_build_mfma(int variant, const float *rows, const uint32_t *perm, const float *norms,
                              const uint8_t *sq8_codes, uint64_t n, void *out, uint8_t *scale, hipStream_t st);
 hipError_t launch_row_norms(const float *rows, uint64_t n, float *out, hipStream_t st);
+// max over rows of |bf16(x) - x| / |x| (FP32, as float bits; rows with |x| = 0
+// count 0) folded into *out_bits with atomicMax (*out_bits must start at 0).
+hipError_t launch_bf16_err_max(const float *rows, uint64_t n, uint32_t *out_bits, hipStream_t st);
 hipError_t launch_gather_f32(const float *src, const uint32_t *perm, uint64_t n, uint64_t n_pad, float *dst,
                              hipStream_t st);
 hipError_t launch_absmax(const float *x, uint64_t count, uint32_t *out_bits, hipStream_t st);

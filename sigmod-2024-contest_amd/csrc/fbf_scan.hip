@@ -278,19 +278,31 @@ __global__ __launch_bounds__(64) void merge_kernel(MergeArgs a) {
         if (a.cert) {
             // Rows outside the candidates have BF16 scores at or above the
             // k'-th one, ts (keys are unique, so "outside" means "key above
-            // tau").  A BF16 score |x|^2_hi + |x|^2_lo - 2 bf16(q).bf16(x)
-            // (FP32 accumulation of 102 exact products) differs from
-            // |x|^2 - 2 q.x by at most 2 |q| |x| (2^-8 + 2^-18 + 112 * 2^-24)
-            // + |x|^2 (2^-18 + 214 * 2^-24) <= |q| |x| 0.00784 + |x|^2 1.7e-5;
-            // the canonical FP32 distances and |q|^2 carry relative errors
-            // below 1e-5.  Fewer than k' candidates (tau = kKeyMax) means the
-            // range held no more rows.
+            // tau").  The tile forms a row's score as |x|^2_hi + |x|^2_lo +
+            // sum_i bf16(x_i) (-2 bf16(q_i)) (exact products, FP32 sums).  With
+            // qb = bf16(q), xb = bf16(x): qb.xb - q.x = (qb - q).xb + q.(xb - x),
+            // so |error of -2 qb.xb| <= 2 |x| (|qb - q| (1 + rho) + |q| rho),
+            // rho >= |xb - x| / |x| over the rows.  The hi + lo split of the
+            // FP32 norm is within 2^-16 |x|^2 of it and the FP32 norm within
+            // 100 * 2^-24 of |x|^2 (2.2e-5 together); summing 102 terms in
+            // FP32 adds at most 111 * 2^-24 of their absolute sum (<= 2.02 |q|
+            // |x| + 1.01 |x|^2); 1.4e-5, twice that, also covers an accumulator
+            // that truncates.  |x| <= cert_norm.  The canonical
+            // FP32 distances and |q|^2 carry relative errors below 1e-5.  Fewer
+            // than k' candidates (tau = kKeyMax) means the range held no more
+            // rows.
             uint32_t ok = 1u;
             if (tau != kKeyMax && cnt >= a.k) {
-                float qn = 0.f;
-                for (int d = 0; d < kDim; ++d) qn = __builtin_fmaf(qv[d], qv[d], qn);
-                const float xm = a.cert_norm;
-                const float eps = 1.01f * (__builtin_sqrtf(qn) * xm * 0.00784f + xm * xm * 1.7e-5f);
+                float qn = 0.f, dq2 = 0.f;
+                for (int d = 0; d < kDim; ++d) {
+                    const float v = qv[d], dv = bf16_round(v) - v;
+                    qn = __builtin_fmaf(v, v, qn);
+                    dq2 = __builtin_fmaf(dv, dv, dq2);
+                }
+                const float xm = a.cert_norm, rho = a.cert_rho;
+                const float qa = __builtin_sqrtf(qn) * 1.0001f, dq = __builtin_sqrtf(dq2) * 1.0001f;
+                const float eps = 1.01f * (2.0f * xm * (dq * (1.0f + rho) + qa * rho) +
+                                           1.4e-5f * (2.02f * qa * xm + 1.01f * xm * xm) + 2.2e-5f * xm * xm);
                 const float ts = ord2f((uint32_t)(tau >> 32));
                 const float lower = (ts + qn * (1.0f - 2e-5f) - eps) * (1.0f - 3e-5f);
                 const float dk = ord2f((uint32_t)(list[a.k - 1] >> 32));

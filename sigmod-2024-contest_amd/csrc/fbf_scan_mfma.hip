@@ -304,6 +304,26 @@ hipError_t launch_build_mfma(int variant, const float *rows, const uint32_t *per
     return hipGetLastError();
 }
 
+__global__ void bf16_err_max_kernel(const float *__restrict__ rows, uint64_t n, uint32_t *__restrict__ out_bits) {
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const float *x = rows + r * kDim;
+    float e2 = 0.f, x2 = 0.f;
+    for (int d = 0; d < kDim; ++d) {
+        const float v = x[d], dv = bf16_round(v) - v;
+        e2 = __builtin_fmaf(dv, dv, e2);
+        x2 = __builtin_fmaf(v, v, x2);
+    }
+    const float rho = x2 > 0.f ? __builtin_sqrtf(e2 / x2) : 0.f;
+    atomicMax(out_bits, __builtin_bit_cast(uint32_t, rho));
+}
+
+hipError_t launch_bf16_err_max(const float *rows, uint64_t n, uint32_t *out_bits, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(bf16_err_max_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, rows, n, out_bits);
+    return hipGetLastError();
+}
+
 hipError_t launch_row_norms(const float *rows, uint64_t n, float *out, hipStream_t st) {
     hipLaunchKernelGGL(row_norms_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, rows, n, out);
     return hipGetLastError();

@@ -1,18 +1,23 @@
 """The certificate of FBF_MODE_EXACT_BF16, checked on the CPU.
 
-1. Its rounding-error bound: a BF16 score as the MFMA scan forms it --
-   |x|^2 (FP32, sequential FMA) split into hi + lo bf16, rows and queries
-   rounded to bf16 (round to nearest even), -2 bf16(q) . bf16(x) plus hi and lo
-   summed in FP32 in any order -- lies within |q||x| 0.00784 + |x|^2 1.7e-5 of
-   |x|^2 - 2 q.x.  Checked for several summation orders on random and on
-   adversarial inputs (aligned vectors, values on bf16 rounding midpoints).
+1. Its rounding-error bound.  The BF16 tile forms a row's score as |x|^2
+   (FP32, sequential FMA) split into hi + lo bf16, plus sum_i bf16(x_i)
+   (-2 bf16(q_i)), summed in FP32.  With qb = bf16(q), xb = bf16(x) and
+   rho_x = |xb - x| / |x|, the bound the merge kernel applies is
+       2 |x| (|qb - q| (1 + rho) + |q| rho) + 1.4e-5 (2.02 |q||x| + 1.01 |x|^2)
+       + 2.2e-5 |x|^2
+   (the kernel uses the dataset's largest rho and |x|, so checking it per row
+   with the row's own rho_x and |x| is the stronger statement).  Checked for
+   several FP32 summation orders on random inputs and on the worst case of
+   bf16 rounding: aligned vectors whose every element lies on the bf16 tie
+   just above a power of two (relative error 2^-8 in both operands, all of
+   one sign), where the error comes within 10% of the bound (and exceeds the
+   constant a 2^-9 unit roundoff would give).
 2. The certificate itself: on random batches with a small k', every query the
    test certifies has the same top-k as a brute-force search over all rows
    with canonical FP32 distances (the exact mode's), ties to the smaller id.
 """
 import numpy as np
-
-BOUND_QX, BOUND_XX = 0.00784, 1.7e-5
 
 
 def bf16(a):
@@ -57,12 +62,20 @@ def sum_f32_pairwise(t):
     return v[0]
 
 
+def row_bound(x, q):
+    xr, qr = x.astype(np.float64), q.astype(np.float64)
+    nx = np.sqrt((xr * xr).sum(1))
+    rho = np.sqrt(((bf16(x).astype(np.float64) - xr) ** 2).sum(1)) / nx
+    nq = np.linalg.norm(qr)
+    dq = np.linalg.norm(bf16(q).astype(np.float64) - qr)
+    return 2 * nx * (dq * (1 + rho) + nq * rho) + 1.4e-5 * (2.02 * nq * nx + 1.01 * nx * nx) + 2.2e-5 * nx * nx
+
+
 def check_bound(x, q):
     t = bf16_terms(x, q)
     xr, qr = x.astype(np.float64), q.astype(np.float64)
     real = (xr * xr).sum(1) - 2.0 * xr @ qr
-    nx = np.sqrt((xr * xr).sum(1))
-    bound = BOUND_QX * np.linalg.norm(qr) * nx + BOUND_XX * nx * nx
+    bound = row_bound(x, q)
     rng = np.random.default_rng(3)
     orders = [np.arange(102), np.arange(102)[::-1], rng.permutation(102)]
     worst = 0.0
@@ -80,18 +93,22 @@ def test_bound_random():
     assert check_bound(x, q) < 1.0
 
 
-def test_bound_adversarial():
+def test_bound_worst_case_rounding():
     rng = np.random.default_rng(1)
-    # Rows aligned with the query (every product positive), values on bf16
-    # rounding midpoints (low 16 bits 0x8000: the worst relative rounding).
-    base = np.abs(rng.standard_normal(100)).astype(np.float32) + 1.0
-    mid = (base.view(np.uint32) & np.uint32(0xFFFF0000)) | np.uint32(0x8000)
-    q = mid.view(np.float32)
-    x = np.repeat(q[None, :], 64, axis=0) * rng.uniform(0.5, 2.0, (64, 1)).astype(np.float32)
-    xm = (x.view(np.uint32) & np.uint32(0xFFFF0000)) | np.uint32(0x7FFF)
-    x = np.concatenate([x, xm.view(np.float32), -x], axis=0)
-    worst = check_bound(x, q)
-    assert 0.2 < worst < 1.0  # the bound is not loose by orders of magnitude here
+    # Every element 2^e (1 + 2^-8): a bf16 tie, rounded to even (down) by a
+    # relative 2^-8, the largest relative rounding error of bf16.
+    e = rng.integers(-3, 4, 100)
+    q = (np.ldexp(1.0 + 2.0 ** -8, e)).astype(np.float32)
+    assert np.all(bf16(q) < q)
+    x = np.repeat(q[None, :], 8, axis=0) * np.ldexp(1.0, rng.integers(-2, 3, (8, 1))).astype(np.float32)
+    worst = check_bound(x.astype(np.float32), q)
+    assert worst > 0.9  # the bound is tight here: rounding errors all of one sign
+    # The constant a 2^-9 unit roundoff would give falls short on these rows.
+    t = bf16_terms(x.astype(np.float32), q)
+    xr, qr = x.astype(np.float64), q.astype(np.float64)
+    err = np.abs(sum_f32(t, np.arange(102)).astype(np.float64) - ((xr * xr).sum(1) - 2 * xr @ qr))
+    naive = 0.00784 * np.linalg.norm(qr) * np.sqrt((xr * xr).sum(1)) + 1.7e-5 * (xr * xr).sum(1)
+    assert np.any(err > naive)
 
 
 def canon_dist(x, q):
@@ -108,7 +125,10 @@ def test_certificate_implies_exact_topk():
     n, k, kp = 6000, 10, 24
     centers = rng.standard_normal((16, 100)).astype(np.float32)
     x = (centers[rng.integers(0, 16, n)] + 0.3 * rng.standard_normal((n, 100))).astype(np.float32)
-    xm = np.float32(np.sqrt(float(fma_seq_norm(x).max())) * (1 + 1e-5))
+    f32 = np.float32
+    xm = f32(np.sqrt(float(fma_seq_norm(x).max())) * (1 + 1e-5))
+    xr = x.astype(np.float64)
+    rho = f32(np.max(np.sqrt(((bf16(x).astype(np.float64) - xr) ** 2).sum(1) / (xr * xr).sum(1))) * 1.001 + 1e-7)
     n_cert = 0
     for qi in range(40):
         q = (centers[qi % 16] + 0.3 * rng.standard_normal(100)).astype(np.float32)
@@ -120,11 +140,16 @@ def test_certificate_implies_exact_topk():
         cd = d[cand]
         o2 = np.lexsort((cand, cd))
         dk = cd[o2[k - 1]]
-        qn = np.float32(0)
+        qn = f32(0)
+        dq2 = f32(0)
         for v in q:
-            qn = np.float32(np.float64(v) * v + qn)
-        eps = np.float32(1.01) * (np.float32(np.sqrt(qn)) * xm * np.float32(BOUND_QX) + xm * xm * np.float32(BOUND_XX))
-        lower = (ts + qn * np.float32(1 - 2e-5) - eps) * np.float32(1 - 3e-5)
+            qn = f32(np.float64(v) * v + qn)
+            dv = np.float64(bf16(np.array([v]))[0]) - v
+            dq2 = f32(dv * dv + dq2)
+        qa, dq = f32(np.sqrt(qn) * 1.0001), f32(np.sqrt(dq2) * 1.0001)
+        eps = f32(1.01) * (f32(2) * xm * (dq * (f32(1) + rho) + qa * rho)
+                           + f32(1.4e-5) * (f32(2.02) * qa * xm + f32(1.01) * xm * xm) + f32(2.2e-5) * xm * xm)
+        lower = (ts + qn * f32(1 - 2e-5) - eps) * f32(1 - 3e-5)
         if lower > dk:
             n_cert += 1
             full = np.lexsort((np.arange(n), d))[:k]
