@@ -12,7 +12,7 @@ the router and saves the output; here every query goes through fbf_search
 ground-truth generator writes).
 
     python3 contest.py --data dummy-data.bin --queries dummy-queries.bin --out output.bin \\
-        [--mode exact|bf16|sq8|fp8|sq8sym] [--k 100] [--gt gt.bin]
+        [--mode exact|exact_bf16|bf16|sq8|fp8|sq8sym] [--k 100] [--gt gt.bin]
 """
 from __future__ import annotations
 
@@ -73,7 +73,7 @@ def main(argv=None) -> int:
     p.add_argument("--data", required=True)
     p.add_argument("--queries", required=True)
     p.add_argument("--out", required=True)
-    p.add_argument("--mode", default="exact", choices=["exact", "bf16", "sq8", "fp8", "sq8sym"])
+    p.add_argument("--mode", default="exact", choices=["exact", "exact_bf16", "bf16", "sq8", "fp8", "sq8sym"])
     p.add_argument("--k", type=int, default=100)
     p.add_argument("--gt", default=None)
     a = p.parse_args(argv)

@@ -139,11 +139,13 @@ def test_host_entry_point_and_filter_count(pkg, orc, torch_cuda, dummy):
     assert np.array_equal(ids, rid) and np.array_equal(d, rd)
 
 
-def test_contest_driver_on_reference_fixture(pkg, golden_dir, tmp_path):
+@pytest.mark.parametrize("mode", ["exact", "exact_bf16"])
+def test_contest_driver_on_reference_fixture(pkg, golden_dir, tmp_path, mode):
     """The contest-format driver end to end: the reference's own dummy base and
     query files in, the contest output file out, equal to the reference's
-    ground truth for those files."""
-    _say("case: contest driver, reference dummy fixture, exact mode")
+    ground truth for those files -- in the exact FP32 mode and in the certified
+    BF16 exact mode."""
+    _say(f"case: contest driver, reference dummy fixture, {mode} mode")
     import importlib.util
     spec = importlib.util.spec_from_file_location("fbf_contest", os.path.join(pkg.__path__[0], "contest.py"))
     contest = importlib.util.module_from_spec(spec)
@@ -151,6 +153,6 @@ def test_contest_driver_on_reference_fixture(pkg, golden_dir, tmp_path):
     out = str(tmp_path / "output.bin")
     gt = os.path.join(golden_dir, "dummy_gt_ref.bin")
     res = contest.run(os.path.join(golden_dir, "dummy-data.bin"), os.path.join(golden_dir, "dummy-queries.bin"),
-                      out, mode="exact", k=100, gt=gt, log=_say)
+                      out, mode=mode, k=100, gt=gt, log=_say)
     assert res["recall"] == 1.0
     assert np.array_equal(pkg.datagen.read_knn_bin(out), pkg.datagen.read_gt_bin(gt))
