@@ -87,7 +87,11 @@ typedef struct fbf_search_opts {
     uint32_t prepass_margin; /* percent of k'/prepass_div keys the prefix pass keeps; its last
                                 kept key is the seed (0 -> default 200).  Queries whose seed proves
                                 too tight are rescanned, so results never depend on it. */
-    uint32_t reserved[1];
+    uint32_t max_batch;   /* fbf_search / fbf_search_host: larger batches run as consecutive
+                             sub-batches of this many queries (0 -> 131072), which bounds the
+                             candidate-list memory (about 0.25 MB per query at the BF16
+                             defaults on 10 M rows); results do not depend on it.  Captured
+                             searches (fbf_search_graph_create) are not split. */
 } fbf_search_opts;
 
 /* Create a device-resident dataset on the current HIP device from host

@@ -1151,6 +1151,18 @@ extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, cons
                                  uint64_t nq, uint32_t k, const fbf_search_opts *opts,
                                  uint32_t *out_ids_dev, float *out_dists_dev, void *stream) {
     if (!ds) return fail(FBF_ERR_INVALID_ARG, "null argument");
+    // Very large batches run as consecutive sub-batches (bounded list memory).
+    const uint64_t max_batch = opts && opts->max_batch ? opts->max_batch : 131072u;
+    if (nq > max_batch) {
+        if (!metas || !queries_dev || !out_ids_dev || !out_dists_dev) return fail(FBF_ERR_INVALID_ARG, "null argument");
+        for (uint64_t i0 = 0; i0 < nq; i0 += max_batch) {
+            const uint64_t m = std::min<uint64_t>(max_batch, nq - i0);
+            const fbf_status s = fbf_search(ds, queries_dev + i0 * ds->d, metas + 4 * i0, m, k, opts,
+                                            out_ids_dev + i0 * k, out_dists_dev + i0 * k, stream);
+            if (s != FBF_OK) return s;
+        }
+        return FBF_OK;
+    }
     if (opts && opts->mode == FBF_MODE_EXACT_BF16)
         return search_exact_bf16(ds, queries_dev, metas, nq, k, *opts, out_ids_dev, out_dists_dev, (hipStream_t)stream);
     {

@@ -50,7 +50,7 @@ class FbfError(RuntimeError):
 class SearchOpts(C.Structure):
     _fields_ = [("mode", C.c_uint32), ("kprime", C.c_uint32), ("chunk_rows", C.c_uint32),
                 ("group_queries", C.c_uint32), ("persistent", C.c_uint32), ("prepass_div", C.c_uint32),
-                ("prepass_margin", C.c_uint32), ("reserved", C.c_uint32 * 1)]
+                ("prepass_margin", C.c_uint32), ("max_batch", C.c_uint32)]
 
 
 _lib = None
@@ -160,7 +160,7 @@ class Dataset:
 
     @staticmethod
     def _opts(mode, kprime, chunk_rows, group_queries, persistent, prepass_div=0,
-              prepass_margin=0) -> SearchOpts:
+              prepass_margin=0, max_batch=0) -> SearchOpts:
         o = SearchOpts()
         o.mode = MODES[mode] if isinstance(mode, str) else int(mode)
         o.kprime = kprime
@@ -169,11 +169,12 @@ class Dataset:
         o.persistent = int(persistent)  # 0 default, 1 (True) persistent, 2 one workgroup per item
         o.prepass_div = prepass_div
         o.prepass_margin = prepass_margin
+        o.max_batch = max_batch
         return o
 
     def search(self, queries, metas: np.ndarray, k: int = 100, mode="exact", kprime: int = 0,
                chunk_rows: int = 0, group_queries: int = 0, persistent: bool = False,
-               out=None, stream=None, prepass_div: int = 0, prepass_margin: int = 0):
+               out=None, stream=None, prepass_div: int = 0, prepass_margin: int = 0, max_batch: int = 0):
         """queries: torch CUDA float32 [nq, d]; metas: numpy float32 [nq, 4].
         Returns torch (ids int32 view of uint32 [nq, k], dists float32 [nq, k])
         on the queries' device; asynchronous on ``stream`` (torch stream or None
@@ -188,7 +189,8 @@ class Dataset:
         else:
             ids, dists = out
         st = (stream or torch.cuda.current_stream(queries.device)).cuda_stream
-        o = self._opts(mode, kprime, chunk_rows, group_queries, persistent, prepass_div, prepass_margin)
+        o = self._opts(mode, kprime, chunk_rows, group_queries, persistent, prepass_div, prepass_margin,
+                       max_batch)
         _check(lib().fbf_search(self.h, _P(_dev_ptr(queries)), _np_ptr(metas), nq, k, C.byref(o),
                                 _P(_dev_ptr(ids)), _P(_dev_ptr(dists)), _P(st)), "fbf_search")
         return ids, dists

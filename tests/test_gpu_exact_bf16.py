@@ -78,3 +78,15 @@ def test_exact_bf16_host_and_errors(pkg, data):
             ds2.search_host(q[:4], metas[:4], k=10, mode="exact_bf16")
     finally:
         ds2.close()
+
+
+@pytest.mark.parametrize("mode", ["exact", "bf16", "exact_bf16"])
+def test_max_batch_split_equals_whole(pkg, data, mode):
+    """A batch larger than max_batch runs as consecutive sub-batches (bounded
+    list memory); the results must not change."""
+    q, metas, ds = data
+    _say(f"case: {mode} whole batch vs max_batch=70")
+    ref_i, ref_d = _search(ds, q, metas, k=100, mode=mode)
+    ids, d = _search(ds, q, metas, k=100, mode=mode, max_batch=70)
+    assert np.array_equal(ids, ref_i)
+    assert np.array_equal(d, ref_d)
