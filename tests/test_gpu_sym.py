@@ -88,3 +88,37 @@ def test_bruteforce_batch(pkg, orc, ref_bf):
         assert np.array_equal(pos[i], sid), i
         valid = sid >= 0
         assert np.array_equal(dist[i][valid], sd[valid])
+
+
+def test_bruteforce_many_threads(pkg, orc, ref_bf):
+    """The router calls bruteforce() from its OpenMP threads at once
+    (hybrid_graph.cpp:168-211): 16 host threads, each with its own stream and
+    scratch, call it concurrently on slices of one registered array; every
+    result must equal the oracle's."""
+    import time
+    from concurrent.futures import ThreadPoolExecutor
+    fbf = pkg.fbf
+    codes = np.ascontiguousarray(ref_bf["codes"])
+    alpha = float(ref_bf["alpha"])
+    rng = np.random.default_rng(5)
+    n_calls = 320
+    starts = rng.integers(0, codes.shape[0] - 1, n_calls)
+    counts = np.minimum(rng.integers(1, 4000, n_calls), codes.shape[0] - starts)
+    qc = orc.sq8sym_encode(rng.standard_normal((n_calls, 100)).astype(np.float32), alpha)
+    _say(f"case: {n_calls} bruteforce calls from 16 threads, registered array")
+    fbf.register_codes(codes)
+    try:
+        def one(i):
+            return fbf.bruteforce(140, int(counts[i]), codes, alpha, qc[i], offset_rows=int(starts[i]))
+        one(0)
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(16) as ex:
+            got = list(ex.map(one, range(n_calls)))
+        dt = time.perf_counter() - t0
+    finally:
+        fbf.unregister_codes(codes)
+    _say(f"  {n_calls / dt:.0f} calls/s")
+    for i in range(n_calls):
+        s, c = int(starts[i]), int(counts[i])
+        sid, _ = orc.bf_sorted(codes[s:s + c], qc[i], 140)
+        assert np.array_equal(got[i], sid), i
