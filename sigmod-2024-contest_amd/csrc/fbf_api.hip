@@ -805,8 +805,10 @@ static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, 
     // thousands an unseeded scan appends while its lists fill.  The merge
     // verifies each seeded query (exact when its k'-th merged key is <= tau0)
     // and the few that fail are rescanned over their whole range.
-    const uint32_t pdiv = mfma ? (o.prepass_div ? o.prepass_div : 8u) : 1u;
-    const uint32_t margin = o.prepass_margin ? o.prepass_margin : 200u;
+    // Defaults 1/10 and 250%: on C1 every MFMA mode runs 0.5-2 ms faster than
+    // with 1/8 and 200%, and no seed fails.
+    const uint32_t pdiv = mfma ? (o.prepass_div ? o.prepass_div : 10u) : 1u;
+    const uint32_t margin = o.prepass_margin ? o.prepass_margin : 250u;
     const uint32_t kpre = (uint32_t)std::min<uint64_t>(K, std::max<uint64_t>(1, ((uint64_t)K * margin + 100ull * pdiv - 1) / (100ull * pdiv)));
     Plan P;
     uint64_t n_items_a = 0;
@@ -1093,6 +1095,10 @@ static fbf_status search_exact_bf16(fbf_dataset *ds, const float *queries_dev, c
     // are certified; a larger k' leaves the 1024-key lists too little room
     // above k' / 2 between compactions, a smaller one certifies fewer.
     if (ob.kprime == 0) ob.kprime = 720;
+    // At this k' a prefix over 1/8 of each range with a 200% margin is
+    // faster than the general defaults (the main pass appends fewer keys).
+    if (ob.prepass_div == 0) ob.prepass_div = 8;
+    if (ob.prepass_margin == 0) ob.prepass_margin = 200;
     // Lists of up to 1024 keys: 256 Ki rows per work item by default keeps
     // their memory (slots x 8 KiB) near 12 GB for 100 k queries over 10 M rows.
     if (ob.chunk_rows == 0) ob.chunk_rows = 262144;
