@@ -965,6 +965,7 @@ static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, 
             a.n_items = i1 - i0;
             a.K = kk;
             a.qmask = mask;
+            a.nmask = mask ? mask + nq : nullptr;  // the verify merge counts the flags there
             a.queue = queue;
             if (queue) {
                 xcd_runs(P.items, i0, i1, mask == nullptr, a.xoff);

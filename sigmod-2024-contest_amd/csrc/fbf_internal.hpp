@@ -139,6 +139,7 @@ struct ScanMfmaArgs {
     uint32_t *queue;
     uint32_t xoff[9];
     const uint32_t *qmask;   // [nq] or null: when set, only queries with qmask[qi] != 0 are scanned
+    const uint32_t *nmask;   // null, or the number of queries set in qmask: the launch exits when 0
 };
 
 struct MergeArgs {

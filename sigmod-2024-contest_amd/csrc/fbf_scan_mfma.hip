@@ -909,6 +909,9 @@ void scan_mfma_kernel(ScanMfmaArgs a) {
     // stream the same rows for different query groups at about the same time
     // and share them in L2), then helps the other runs.  One call site keeps
     // a single inlined copy of the item code.
+    // A masked launch with no query set (the rescan after a verify that
+    // failed no seed) has nothing to do.
+    if (a.nmask && __builtin_amdgcn_readfirstlane(*a.nmask) == 0) return;
     const uint32_t me = xcc_id();
     for (uint32_t round = 0;; ++round) {
         if (threadIdx.x == 0) {
