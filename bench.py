@@ -50,7 +50,7 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--exact-bf16", action=argparse.BooleanOptionalAction, default=True,
                    help="in BF16 runs, also time the certified exact mode (results identical to exact FP32)")
-    p.add_argument("--recall-sample", type=int, default=16, help="queries checked against FP64")
+    p.add_argument("--recall-sample", type=int, default=64, help="queries checked against FP64")
     p.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True,
                    help="pick group_queries / chunk_rows / kprime with the project's autotuner (untimed) "
                         "unless they are given (approximate modes)")
