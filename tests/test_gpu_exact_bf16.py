@@ -90,3 +90,39 @@ def test_max_batch_split_equals_whole(pkg, data, mode):
     ids, d = _search(ds, q, metas, k=100, mode=mode, max_batch=70)
     assert np.array_equal(ids, ref_i)
     assert np.array_equal(d, ref_d)
+
+
+@pytest.mark.parametrize("k", [1, 100, 512])
+def test_edge_filters_all_modes(pkg, k):
+    """Absent category, inverted and single-timestamp windows, ragged N, k up to
+    512: the certified mode equals the exact mode bit for bit; the BF16 mode
+    equals it on every query with no more rows than k' (all rows reranked) and
+    pads the same way; an empty batch is a no-op in every mode."""
+    import torch
+    fbf, dg = pkg.fbf, pkg.datagen
+    cfg = dg.CONFIGS[0]
+    vecs, labels, ts = dg.make_base(cfg, n=3037)
+    q, metas = dg.make_queries(cfg, nq=120)
+    metas[0] = [1, 99999, -1, -1]
+    metas[1] = [2, -1, 0.6, 0.5]
+    metas[2] = [2, -1, ts[11], ts[11]]
+    metas[3] = [3, labels[5], ts[5], ts[5]]
+    _say(f"case: N=3037 edge filters, k={k}, exact / exact_bf16 / bf16")
+    ds = fbf.Dataset(vecs, labels, ts, repr_flags=fbf.REPR_F32 | fbf.REPR_BF16)
+    try:
+        ref_i, ref_d = _search(ds, q, metas, k=k, mode="exact")
+        ids, d = _search(ds, q, metas, k=k, mode="exact_bf16")
+        assert np.array_equal(ids, ref_i) and np.array_equal(d, ref_d)
+        kp = min(896, max(k, (3 * k + 1) // 2))
+        bi, bd = _search(ds, q, metas, k=k, mode="bf16")
+        for i in range(len(q)):
+            c, _ = ds.filter_count(metas[i])
+            if c <= kp:
+                assert np.array_equal(bi[i], ref_i[i]) and np.array_equal(bd[i], ref_d[i]), i
+        assert np.all(bi[0] == fbf.PAD_ID) and np.all(bi[1] == fbf.PAD_ID)
+        empty_q = torch.zeros((0, 100), dtype=torch.float32).cuda()
+        for mode in ("exact", "bf16", "exact_bf16"):
+            i0, d0 = ds.search(empty_q, np.zeros((0, 4), np.float32), k=k, mode=mode)
+            assert i0.shape == (0, k) and d0.shape == (0, k)
+    finally:
+        ds.close()
