@@ -50,7 +50,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--exact-bf16", action=argparse.BooleanOptionalAction, default=True,
                    help="in BF16 runs, also time the certified exact mode (results identical to exact FP32)")
-    p.add_argument("--recall-sample", type=int, default=64, help="queries checked against FP64")
+    p.add_argument("--recall-sample", type=int, default=512,
+                   help="type-stratified queries checked against the FP64 exact answer")
     p.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True,
                    help="pick group_queries / chunk_rows / kprime with the project's autotuner (untimed) "
                         "unless they are given (approximate modes)")
@@ -97,7 +98,9 @@ def main() -> int:
     mode = a.mode
     repr_flags = fbf.MODE_REPR[fbf.MODES[mode]]
     if mode != "exact":
-        repr_flags |= fbf.REPR_F32  # rerank / recall reference
+        # FP32 rows for the rerank; BF16 tiles for the certified exact search
+        # that gives the whole batch's exact answer for the recall check.
+        repr_flags |= fbf.REPR_F32 | fbf.REPR_BF16
     dev = torch.device("cuda:0")
     t0 = time.time()
     ds = fbf.Dataset(vecs, labels, ts, repr_flags=repr_flags)
@@ -150,9 +153,13 @@ def main() -> int:
     ds.profile(False)
     elapsed = t_end - t_start
     stats = ds.last_stats()
+    pairs_prefix, pairs_main = ds.last_pass_pairs()
     rescans = ds.last_rescans()
     fallbacks = ds.last_fallbacks()
-    n_scan, ms_scan = ds.profile_collect("scan")
+    n_main, ms_main = ds.profile_collect("scan_main")
+    n_pre, ms_pre = ds.profile_collect("scan_prefix")
+    n_resc, ms_resc = ds.profile_collect("scan_rescan")
+    n_scan, ms_scan = n_main + n_pre + n_resc, ms_main + ms_pre + ms_resc
     n_merge, ms_merge = ds.profile_collect("merge")
     n_rr, ms_rr = ds.profile_collect("rerank")
     n_plan, ms_plan = ds.profile_collect("plan")
@@ -162,7 +169,8 @@ def main() -> int:
     ms_step = 1000.0 * elapsed / a.steps
     qps = nq * a.steps / elapsed
     log(f"{a.steps} steps in {elapsed:.3f}s: {qps:.0f} QPS; scan {ms_scan / max(1, a.steps):.1f} ms/step "
-        f"({n_scan} launches), merge+rerank {ms_merge / max(1, a.steps):.1f} ms/step, "
+        f"({n_scan} launches; main {ms_main / max(1, a.steps):.1f}, prefix {ms_pre / max(1, a.steps):.1f}, "
+        f"rescan {ms_resc / max(1, a.steps):.2f}), merge+rerank {ms_merge / max(1, a.steps):.1f} ms/step, "
         f"{stats['work_items']} work items, {stats['pairs_computed'] / max(1, stats['pairs']):.3f}x pairs computed, "
         f"{rescans} queries rescanned, host planning {ms_plan / max(1, a.steps):.1f} ms/step"
         + (f", {fallbacks} queries left to the FP32 scan by the certificate" if mode == "exact_bf16" else ""))
@@ -196,10 +204,12 @@ def main() -> int:
     # rerank + certificate + FP32 fallback): exact results, timed like the
     # main line (1 warmup, then min(steps, 3) timed searches).
     exact_cert = None
+    exact_ids = None
     if mode == "bf16" and a.exact_bf16:
         kwe = dict(k=a.k, mode="exact_bf16", out=out)
         ds.search(qd, metas, **kwe)
         torch.cuda.synchronize()
+        exact_ids = out[0].cpu().numpy().view(np.uint32).copy()
         ne = min(a.steps, 3)
         t0 = time.perf_counter()
         for i in range(ne):
@@ -215,8 +225,32 @@ def main() -> int:
         ids = out[0].cpu().numpy().view(np.uint32)
         dists = out[1].cpu().numpy()
 
-    # Recall@k against FP64 exact on a type-stratified sample (CPU oracle).
+    # Recall@k of the timed mode over the WHOLE batch against the exact
+    # answer (the certified exact mode: identical to exact FP32), outside the
+    # timed region.
     orc = ge.load_oracle()
+    full_recall = None
+    if mode not in ("exact", "exact_bf16"):
+        if exact_ids is None:
+            eo = (torch.empty_like(out[0]), torch.empty_like(out[1]))
+            ds.search(qd, metas, k=a.k, mode="exact_bf16", out=eo)
+            torch.cuda.synchronize()
+            exact_ids = eo[0].cpu().numpy().view(np.uint32)
+            del eo
+        full_recall, per_q = orc.recall(ids, exact_ids)
+        by_type = {str(t): float(orc.recall(ids[metas[:, 0] == t], exact_ids[metas[:, 0] == t])[0])
+                   for t in range(4)}
+        worst = int(np.argmin(per_q))
+        full_recall = {"recall": float(full_recall), "queries": int(nq), "by_type": by_type,
+                       "worst_query": worst, "worst_hits": int(per_q[worst]), "reference": "exact_bf16"}
+        log(f"recall@{a.k} vs exact over all {nq} queries: {full_recall['recall']:.5f}; by type {by_type}")
+
+    # Recall@k and distance errors against FP64 exact on a type-stratified
+    # sample (CPU oracle over the sorted ranges).
+    t0 = time.time()
+    lay = orc.Layouts(labels.astype(np.int32), ts)
+    vb = np.ascontiguousarray(vecs[lay.perm_both])
+    vt = np.ascontiguousarray(vecs[lay.perm_time])
     rng = np.random.default_rng(0)
     sample = []
     for t in range(4):
@@ -224,42 +258,65 @@ def main() -> int:
         if idx.size:
             sample += rng.choice(idx, min(idx.size, max(1, a.recall_sample // 4)), replace=False).tolist()
     sample = np.array(sorted(sample))
-    t0 = time.time()
-    gi, gd = orc.exact_predicate(vecs, labels.astype(np.int32), ts, q[sample], metas[sample], k=a.k, mode="f64")
-    rec, _ = orc.recall(ids[sample], gi)
-    valid = np.isfinite(gd) & (gd > 0)
-    kth_err = float(np.max(np.abs(dists[sample][valid] - gd[valid]) / gd[valid])) if valid.any() else 0.0
-    log(f"recall@{a.k} vs FP64 on {sample.size} queries: {rec:.4f}, max rel err {kth_err:.2e} "
-        f"({time.time() - t0:.1f}s)")
+    gi, gd = orc.exact_ranges_f64_prepared(vb, vt, lay, q[sample], metas[sample], a.k)
+    rec, per_s = orc.recall(ids[sample], gi)
+    fp64 = {"queries": int(sample.size), "recall": float(rec), "by_type": {}}
+    kth_err = 0.0
+    for t in range(4):
+        sel = metas[sample, 0] == t
+        if not sel.any():
+            continue
+        # k-th distance relative error: the k-th returned distance against
+        # the FP64 k-th distance (queries with at least k candidates)
+        dk, gk = dists[sample][sel, a.k - 1].astype(np.float64), gd[sel, a.k - 1]
+        ok = np.isfinite(gk) & (gk > 0)
+        e = float(np.max(np.abs(dk[ok] - gk[ok]) / gk[ok])) if ok.any() else 0.0
+        r_t = float(orc.recall(ids[sample][sel], gi[sel])[0])
+        wi = int(np.argmin(per_s[sel]))
+        fp64["by_type"][str(t)] = {"recall": r_t, "kth_rel_err": e, "queries": int(sel.sum()),
+                                   "worst_query": int(sample[sel][wi]), "worst_hits": int(per_s[sel][wi])}
+        kth_err = max(kth_err, e)
+    fp64["kth_rel_err"] = kth_err
+    log(f"recall@{a.k} vs FP64 on {sample.size} stratified queries: {rec:.5f}, max k-th distance rel err "
+        f"{kth_err:.2e}; {fp64['by_type']} ({time.time() - t0:.1f}s)")
 
-    # Roofline of the dominant kernel (the scan), from its measured launches.
+    # Roofline of the dominant kernel, from its launches measured on the
+    # search's own stream (HIP events).  MFMA modes: the main scan pass (one
+    # launch per step; 2d = 200 flop per (query, candidate) pair of that
+    # pass), with the figure over all scan launches beside it.  Exact FP32:
+    # the VALU scan's launches (one per sorted layout), 3d = 300 flop/pair.
     pairs = stats["pairs"]
-    # Launches of one step can differ in size (one per sorted layout for the
-    # VALU scans); the achieved rate is total algorithmic work / total time.
-    avg_launch_ms = ms_scan / max(1, n_scan)
-    launches_per_step = max(1, n_scan // a.steps)
-    pairs_per_launch = pairs / launches_per_step
     if mode in ("exact",):
-        # sub, mul, add per dimension: 3d FP32 flops per (query, row) pair.
-        flops = 3.0 * 100 * pairs_per_launch
+        flops = 3.0 * 100 * pairs / max(1, n_scan // a.steps)
+        avg_launch_ms = ms_scan / max(1, n_scan)
         achieved = flops / (avg_launch_ms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS["fp32"], "unit": "TFLOP/s",
-                "frac": achieved / PEAK_TFLOPS["fp32"], "traffic": committed_traffic(cfg.name, mode, a, kw)}
+        roof = {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS["fp32"], "unit": "TFLOP/s",
+                "frac": achieved / PEAK_TFLOPS["fp32"], "traffic": committed_traffic(cfg.name, mode, a, kw),
+                "kernel": "scan_f32_kernel", "launch_ms": avg_launch_ms}
     else:
-        flops = 2.0 * 100 * pairs_per_launch
-        # exact_bf16: BF16 flops over all its scan launches (the FP32 fallback included).
         pk = PEAK_TFLOPS["bf16"] if mode in ("bf16", "sq8", "exact_bf16") else PEAK_TFLOPS["fp8"] if mode == "fp8" else \
             PEAK_TFLOPS["int8"]
-        achieved = flops / (avg_launch_ms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": achieved, "peak": pk,
-                "unit": "TOP/s" if mode == "sq8sym" else "TFLOP/s",
-                "frac": achieved / pk, "traffic": committed_traffic(cfg.name, mode, a, kw)}
+        unit = "TOP/s" if mode == "sq8sym" else "TFLOP/s"
+        main_ms = ms_main / max(1, n_main)
+        achieved = 2.0 * 100 * pairs_main / (main_ms * 1e-3) / 1e12
+        all_scans = 2.0 * 100 * pairs * a.steps / (ms_scan * 1e-3) / 1e12
+        if mode == "exact_bf16":
+            # its main-pass events also hold the FP32 fallback's scans: use all scans
+            achieved, main_ms = all_scans, ms_scan / max(1, n_scan)
+        roof = {"bound": "mfma", "achieved": achieved, "peak": pk, "unit": unit, "frac": achieved / pk,
+                "traffic": committed_traffic(cfg.name, mode, a, kw), "kernel": "scan_mfma_kernel (main pass)",
+                "launch_ms": main_ms, "pairs_per_launch": pairs_main,
+                "all_scan_launches": {"achieved": all_scans, "frac": all_scans / pk,
+                                      "ms_per_step": ms_scan / a.steps}}
 
     cpu = None
     cpu_router = None
+    cpu_ref = None
     if not a.no_cpu:
-        cpu = cpu_baseline(orc, vecs, labels, ts, q, metas, a.k, a.cpu_seconds)
-        cpu_router = cpu_router_bf(orc, vecs, labels, ts, q, metas, a.k, a.cpu_seconds / 2)
+        cpu = cpu_baseline(orc, vb, vt, lay, q, metas, a.k, a.cpu_seconds)
+        cpu_router = cpu_router_bf(orc, vecs, lay, q, metas, a.k, a.cpu_seconds / 2)
+        cpu_ref = cpu_reference_router(orc, vecs, lay, q, metas, a.k, a.cpu_seconds / 2)
+    del vb, vt
 
     line = {
         "metric": "filtered top-100 QPS at recall@100>=0.99 (BF16/SQ8 vs FP32), 1 MI355X",
@@ -283,10 +340,13 @@ def main() -> int:
                    "certified_exact": exact_cert},
         "recall_at_k_vs_fp64": rec,
         "recall_sample": int(sample.size),
-        "max_rel_err_vs_fp64": kth_err,
+        "kth_rel_err_vs_fp64": kth_err,
+        "fp64_check": fp64,
+        "recall_full_batch_vs_exact": full_recall,
         "roofline": roof,
         "cpu_baseline": cpu,
         "cpu_baseline_router_bf": cpu_router,
+        "cpu_baseline_reference_router_bf": cpu_ref,
     }
     print(json.dumps(line), flush=True)
     ds.close()
@@ -320,15 +380,29 @@ def committed_traffic(workload, mode, a, kw):
     return traffic if profiled == opts else None
 
 
-def cpu_baseline(orc, vecs, labels, ts, q, metas, k, target_s):
+def host_cpu() -> dict:
+    """The host's logical CPU count, the CPUs this process may use, and the
+    CPU model (lscpu's "Model name", from /proc/cpuinfo)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return {"nproc": usable, "logical_cpus": os.cpu_count(), "cpu_model": model}
+
+
+def cpu_baseline(orc, vb, vt, L, q, metas, k, target_s):
     """Exact canonical-FP32 top-k over the sorted contiguous ranges (the CPU
     port of the path), all host threads, on a type-stratified sample sized to
     about target_s seconds; QPS extrapolates over the sample."""
-    t0 = time.time()
-    L = orc.Layouts(labels.astype(np.int32), ts)
-    vb = np.ascontiguousarray(vecs[L.perm_both])
-    vt = np.ascontiguousarray(vecs[L.perm_time])
-    log(f"cpu baseline layouts in {time.time() - t0:.1f}s")
     rng = np.random.default_rng(1)
     order = np.concatenate([rng.permutation(np.nonzero(metas[:, 0] == t)[0]) for t in range(4)])
     # interleave types so every prefix is stratified
@@ -344,19 +418,18 @@ def cpu_baseline(orc, vecs, labels, ts, q, metas, k, target_s):
         done += idx.size
         size = min(size * 2, 4096)
     threads = orc.num_threads()
-    log(f"cpu baseline: {done} queries in {spent:.1f}s on {threads} threads")
-    return {"value": done / spent, "unit": "queries/s", "cores": threads, "kind": "port",
-            "sample": f"{done} type-stratified queries of the same workload, exact canonical FP32 over "
-                      f"sorted ranges (oracle/fbf_oracle.c orc_exact_ranges, OpenMP)"}
+    log(f"cpu baseline: {done} queries in {spent:.1f}s on {threads} threads ({host_cpu()})")
+    return dict({"value": done / spent, "unit": "queries/s", "cores": threads, "kind": "port",
+                 "sample": f"{done} type-stratified queries of the same workload, exact canonical FP32 over "
+                           f"sorted ranges (oracle/fbf_oracle.c orc_exact_ranges, OpenMP)"}, **host_cpu())
 
 
-def cpu_router_bf(orc, vecs, labels, ts, q, metas, k, target_s):
+def cpu_router_bf(orc, vecs, L, q, metas, k, target_s):
     """The reference's own brute-force route on the CPU (hybrid_graph.cpp:
     203-228 restated in the oracle, orc_router_bf): symmetric int8 codes,
     top-140 scan of the filtered range, exact FP32 rerank to k, all host
     threads, on a type-stratified sample sized to about target_s seconds."""
     t0 = time.time()
-    L = orc.Layouts(labels.astype(np.int32), ts)
     alpha = orc.sq8sym_train(vecs)
     codes = orc.sq8sym_encode(vecs, alpha)
     cb = np.ascontiguousarray(codes[L.perm_both])
@@ -384,9 +457,50 @@ def cpu_router_bf(orc, vecs, labels, ts, q, metas, k, target_s):
         size = min(size * 2, 4096)
     threads = orc.num_threads()
     log(f"cpu router path (SQ8-sym top-140 + FP32 rerank): {done} queries in {spent:.1f}s on {threads} threads")
-    return {"value": done / spent, "unit": "queries/s", "cores": threads, "kind": "port",
-            "sample": f"{done} type-stratified queries; the reference's brute-force route (symmetric int8 "
-                      f"top-140 scan + exact FP32 rerank, oracle/fbf_oracle.c orc_router_bf, OpenMP)"}
+    return dict({"value": done / spent, "unit": "queries/s", "cores": threads, "kind": "port",
+                 "sample": f"{done} type-stratified queries; the reference's brute-force route (symmetric int8 "
+                           f"top-140 scan + exact FP32 rerank, oracle/fbf_oracle.c orc_router_bf, OpenMP)"},
+                **host_cpu())
+
+
+def cpu_reference_router(orc, vecs, L, q, metas, k, target_s):
+    """The reference's own kernels (bruteforce(), L2SqrSQ8_sym, L2SqrRef and
+    maxPQIFCS, compiled from its headers into oracle/_ref) driven as its
+    router drives them (hybrid_graph.cpp:168-228, one OpenMP task per query),
+    on a type-stratified sample; None when that library is absent or the host
+    lacks the AVX-512 subsets it was built for."""
+    R = orc.ref()
+    if R is None:
+        log("reference kernels not available on this host: no reference-kind CPU baseline")
+        return None
+    alpha = orc.sq8sym_train(vecs)
+    codes = orc.sq8sym_encode(vecs, alpha)
+    cb = np.ascontiguousarray(codes[L.perm_both])
+    ct = np.ascontiguousarray(codes[L.perm_time])
+    del codes
+    rng = np.random.default_rng(3)
+    per = [rng.permutation(np.nonzero(metas[:, 0] == t)[0]) for t in range(4)]
+    m = min(len(p) for p in per)
+    order = np.stack([p[:m] for p in per], 1).reshape(-1)
+    done, spent, size = 0, 0.0, 4
+    while done < order.size and spent < target_s:
+        idx = np.sort(order[done:done + size])
+        qq = np.ascontiguousarray(q[idx])
+        rr = np.ascontiguousarray(L.ranges(metas[idx]).astype(np.int64))
+        qc = orc.sq8sym_encode(qq, alpha)
+        ids = np.zeros((idx.size, k), np.uint32)
+        t = time.perf_counter()
+        R.ref_router_bf(orc._p(cb), orc._p(ct), orc._p(L.perm_both), orc._p(L.perm_time), orc._p(vecs), 100,
+                        alpha, orc._p(qq), orc._p(qc), orc._p(rr), idx.size, 140, k, orc._p(ids))
+        spent += time.perf_counter() - t
+        done += idx.size
+        size = min(size * 2, 4096)
+    threads = orc.num_threads()
+    log(f"reference kernels, router route: {done} queries in {spent:.1f}s on {threads} threads")
+    return dict({"value": done / spent, "unit": "queries/s", "cores": threads, "kind": "reference",
+                 "sample": f"{done} type-stratified queries; the reference's own bruteforce() top-140 + "
+                           f"L2SqrRef rerank (oracle/_ref, built from its sources), OpenMP over queries"},
+                **host_cpu())
 
 
 if __name__ == "__main__":

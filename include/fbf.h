@@ -153,6 +153,17 @@ fbf_status fbf_search_graph_destroy(fbf_search_graph *g);
  * sq8_quant.hpp:30-47).  mi and dif receive d floats each (may be NULL). */
 fbf_status fbf_dataset_quant_params(const fbf_dataset *ds, float *alpha, float *mi, float *dif);
 
+/* Rows [row0, row0 + nrows) of one sorted layout (0 = (label, timestamp, id)
+ * order, 1 = (timestamp, id) order) as the dataset holds them: the original id
+ * at each position (ids, may be NULL) and, when codes is not NULL, the codes
+ * of representation repr -- FBF_REPR_SQ8SYM: 112-byte symmetric int8 codes
+ * (the router's codes_both / codes_time, hybrid_graph.cpp:515-522, encoded as
+ * SQ8SymmetricQuantizer::encode, sq8_sym_quant.hpp:63-74); FBF_REPR_SQ8: 100
+ * per-dimension uint8 codes per row (SQ8Quantizer::encode, sq8_quant.hpp:51-63).
+ * Host arrays; waits for the device. */
+fbf_status fbf_dataset_layout_rows(const fbf_dataset *ds, uint32_t layout, uint64_t row0, uint64_t nrows,
+                                   uint32_t repr, uint32_t *ids, void *codes);
+
 /* ---------------------------------------------------------------------
  * Reference-shaped entry points (bruteforce.h:244-249 and :267-273).
  *
@@ -195,7 +206,9 @@ fbf_status fbf_bruteforce_batch(const char *codes_base, const uint64_t *starts,
 /* When enabled, every kernel launch of this dataset is bracketed by HIP
  * events on its stream; fbf_profile_collect waits for them and returns the
  * launch count and summed milliseconds of the kernel named `kernel`
- * ("scan", "merge", "rerank", "build", or "all"), then clears the log.
+ * ("scan", "merge", "rerank", "build", or "all"; "scan_main", "scan_prefix"
+ * and "scan_rescan" select the MFMA scans' passes: main (or only) pass,
+ * seeded prefix pass, masked rescan), then clears the entries it returns.
  * "plan" returns the host time fbf_search spent planning the work items and
  * queueing their upload (one entry per search). */
 fbf_status fbf_profile_enable(fbf_dataset *ds, int enable);
@@ -207,6 +220,11 @@ fbf_status fbf_profile_collect(fbf_dataset *ds, const char *kernel, uint64_t *la
  * (including tile padding), and the number of work items. */
 fbf_status fbf_last_search_stats(const fbf_dataset *ds, uint64_t *pairs, uint64_t *pairs_computed,
                                  uint64_t *work_items);
+
+/* Algorithmic (query, candidate) pairs of the last fbf_search's seeded
+ * prefix pass and of its main pass (prefix 0 when it ran none); they sum to
+ * fbf_last_search_stats' pairs.  No reference counterpart: measurement. */
+fbf_status fbf_last_search_pass_pairs(const fbf_dataset *ds, uint64_t *prefix_pairs, uint64_t *main_pairs);
 
 /* Number of queries of the last fbf_search on this dataset whose seeded
  * threshold failed verification and were rescanned over their whole range

@@ -436,6 +436,60 @@ void orc_exact_ranges(const float *vec_b, const uint32_t *id_b, const int32_t *l
     }
 }
 
+/* The same search in FP64 (check time only: recall and k-th distance error
+ * against the exact answer): each range's k smallest (FP64 distance, id),
+ * ascending, padded with id 0xFFFFFFFF and +inf.  Equal to orc_exact_predicate
+ * mode 1 on every query (tests/test_oracle.py). */
+void orc_exact_ranges_f64(const float *vec_b, const uint32_t *id_b, const int32_t *lab_b,
+                          const float *ts_b, const float *vec_t, const uint32_t *id_t,
+                          const float *ts_t, uint64_t n, int d, const float *queries,
+                          const float *metas, uint64_t nq, int k, uint32_t *out_ids,
+                          double *out_dists) {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t qi = 0; qi < (int64_t)nq; ++qi) {
+        int lay; uint64_t s, e;
+        orc_query_range((int)metas[qi * 4], (int32_t)metas[qi * 4 + 1], metas[qi * 4 + 2],
+                        metas[qi * 4 + 3], lab_b, ts_b, ts_t, n, &lay, &s, &e);
+        const float *vec = lay == 0 ? vec_b : vec_t;
+        const uint32_t *idm = lay == 0 ? id_b : id_t;
+        const float *q = queries + qi * d;
+        double *bd = (double *)malloc(sizeof(double) * (k + 1));
+        uint32_t *bi = (uint32_t *)malloc(sizeof(uint32_t) * (k + 1));
+        int cnt = 0;
+        for (uint64_t j = s; j < e; ++j) {
+            const double dist = orc_l2_f64(vec + j * d, q, d);
+            const uint32_t id = idm[j];
+            /* keep the k smallest by (dist, id): positions are not in id order */
+            if (cnt == k && !(dist < bd[k - 1] || (dist == bd[k - 1] && id < bi[k - 1]))) continue;
+            int p = cnt < k ? cnt : k - 1;
+            while (p > 0 && (bd[p - 1] > dist || (bd[p - 1] == dist && bi[p - 1] > id))) {
+                bd[p] = bd[p - 1]; bi[p] = bi[p - 1]; --p;
+            }
+            bd[p] = dist; bi[p] = id;
+            if (cnt < k) ++cnt;
+        }
+        for (int t = 0; t < k; ++t) {
+            out_ids[qi * k + t] = t < cnt ? bi[t] : ORC_PAD_ID;
+            out_dists[qi * k + t] = t < cnt ? bd[t] : HUGE_VAL;
+        }
+        free(bd); free(bi);
+    }
+}
+
+/* Canonical FP32 distance of every returned (query, id) pair: out[q*k + j] =
+ * orc_l2_canon(base[ids[q*k + j]], queries[q]), +inf for the pad id.  base is
+ * row-major in original id order. */
+void orc_l2_canon_rows(const float *base, int d, const float *queries, const uint32_t *ids,
+                       uint64_t nq, int k, float *out) {
+#pragma omp parallel for schedule(static)
+    for (int64_t qi = 0; qi < (int64_t)nq; ++qi)
+        for (int j = 0; j < k; ++j) {
+            const uint32_t id = ids[qi * k + j];
+            out[qi * k + j] = id == ORC_PAD_ID ? HUGE_VALF
+                                               : orc_l2_canon(base + (uint64_t)id * d, queries + qi * d, d);
+        }
+}
+
 /* ------------------------------------------------------------------------ */
 /* reference bruteforce() semantics (bruteforce.h:51-117, 244-289)           */
 /* ------------------------------------------------------------------------ */

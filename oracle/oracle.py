@@ -74,6 +74,8 @@ def lib():
         L.orc_query_range.argtypes = [_I, C.c_int32, _F, _F, _P, _P, _P, _U64,
                                       C.POINTER(_I), C.POINTER(_U64), C.POINTER(_U64)]
         L.orc_exact_ranges.argtypes = [_P, _P, _P, _P, _P, _P, _P, _U64, _I, _P, _P, _U64, _I, _P, _P]
+        L.orc_exact_ranges_f64.argtypes = [_P, _P, _P, _P, _P, _P, _P, _U64, _I, _P, _P, _U64, _I, _P, _P]
+        L.orc_l2_canon_rows.argtypes = [_P, _I, _P, _P, _U64, _I, _P]
         L.orc_bf_heap_ref.argtypes = [_I, _U64, _P, _P, _P, _P]
         L.orc_bf_sorted.argtypes = [_I, _U64, _P, _P, _P, _P]
         L.orc_router_bf.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P, _U64, _I, _F, _P, _P, _U64,
@@ -278,6 +280,27 @@ def exact_ranges_prepared(vb, vt, lay: Layouts, queries, metas, k=100):
     return ids, dists
 
 
+def exact_ranges_f64_prepared(vb, vt, lay: Layouts, queries, metas, k=100):
+    """FP64 exact top-k over the sorted ranges (check time only).  Returns
+    (ids uint32, dists float64), ascending by (distance, id), padded."""
+    queries = np.ascontiguousarray(queries, np.float32); metas = np.ascontiguousarray(metas, np.float32)
+    nq = queries.shape[0]
+    ids = np.zeros((nq, k), np.uint32); dists = np.zeros((nq, k), np.float64)
+    lib().orc_exact_ranges_f64(_p(vb), _p(lay.perm_both), _p(lay.lab_b), _p(lay.ts_b), _p(vt),
+                               _p(lay.perm_time), _p(lay.ts_t), lay.n, vb.shape[1], _p(queries),
+                               _p(metas), nq, k, _p(ids), _p(dists))
+    return ids, dists
+
+
+def l2_canon_rows(base, queries, ids) -> np.ndarray:
+    """Canonical FP32 distance of each (query, returned id); +inf for the pad id."""
+    base = np.ascontiguousarray(base, np.float32); queries = np.ascontiguousarray(queries, np.float32)
+    ids = np.ascontiguousarray(ids, np.uint32)
+    out = np.zeros(ids.shape, np.float32)
+    lib().orc_l2_canon_rows(_p(base), base.shape[1], _p(queries), _p(ids), ids.shape[0], ids.shape[1], _p(out))
+    return out
+
+
 # ---------------------------------------------------------------- reference bf semantics
 def bf_heap_ref(codes: np.ndarray, qcode: np.ndarray, k: int):
     """maxPQIFCS-driven bruteforce()/bruteforce_subgraph() (bruteforce.h:244-289):
@@ -293,6 +316,35 @@ def bf_sorted(codes: np.ndarray, qcode: np.ndarray, k: int):
     ids = np.zeros(k, np.int32); d = np.zeros(k, np.int32)
     lib().orc_bf_sorted(k, codes.shape[0], _p(codes), _p(qcode), _p(ids), _p(d))
     return ids, d
+
+
+def router_bf(base, lay: Layouts, codes_b, codes_t, alpha, queries, metas, kq=140, k=100):
+    """The router's brute-force branch (hybrid_graph.cpp:203-228) restated
+    (orc_router_bf): symmetric int8 top-kq of each query's range, ties to the
+    smaller position, id map, canonical FP32 rerank to k.  Returns (ids, dists)."""
+    base = np.ascontiguousarray(base, np.float32)
+    queries = np.ascontiguousarray(queries, np.float32); metas = np.ascontiguousarray(metas, np.float32)
+    nq = queries.shape[0]
+    ids = np.zeros((nq, k), np.uint32); dists = np.zeros((nq, k), np.float32)
+    lib().orc_router_bf(_p(codes_b), _p(codes_t), _p(lay.perm_both), _p(lay.lab_b), _p(lay.ts_b),
+                        _p(lay.perm_time), _p(lay.ts_t), _p(base), base.shape[0], base.shape[1], alpha,
+                        _p(queries), _p(metas), nq, kq, k, _p(ids), _p(dists))
+    return ids, dists
+
+
+def bf_boundary_tied(lay: Layouts, codes_b, codes_t, qcodes, metas, kq=140) -> np.ndarray:
+    """True for queries whose range holds more than kq rows and whose kq-th and
+    (kq+1)-th smallest symmetric int8 distances are equal: there the
+    reference's heap may keep a different row than the smaller position."""
+    out = np.zeros(metas.shape[0], bool)
+    for i, m in enumerate(metas):
+        layout, s, e = lay.query_range(int(m[0]), int(m[1]), m[2], m[3])
+        if e - s <= kq:
+            continue
+        c = codes_b if layout == 0 else codes_t
+        _, d = bf_sorted(c[s:e], qcodes[i], kq + 1)
+        out[i] = d[kq - 1] == d[kq]
+    return out
 
 
 def recall(knn: np.ndarray, gt: np.ndarray):

@@ -43,12 +43,15 @@ static fbf_status fail(fbf_status s, const std::string &msg) {
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    // Grows to at least b bytes, by 1.5x of the old size or more, so a run of
+    // slightly larger requests does not reallocate each time.
     hipError_t reserve(size_t b) {
         if (b <= bytes) return hipSuccess;
+        const size_t old = bytes;
         if (p) hipFree(p);
         p = nullptr;
         bytes = 0;
-        size_t nb = std::max(b, bytes + bytes / 2);
+        const size_t nb = std::max(b, old + old / 2);
         hipError_t e = hipMalloc(&p, nb);
         if (e == hipSuccess) bytes = nb;
         return e;
@@ -66,10 +69,11 @@ struct HostPinned {
     size_t bytes = 0;
     hipError_t reserve(size_t b) {
         if (b <= bytes) return hipSuccess;
+        const size_t old = bytes;
         if (p) hipHostFree(p);
         p = nullptr;
         bytes = 0;
-        size_t nb = std::max(b, bytes + bytes / 2);
+        const size_t nb = std::max(b, old + old / 2);
         hipError_t e = hipHostMalloc(&p, nb, hipHostMallocDefault);
         if (e == hipSuccess) bytes = nb;
         return e;
@@ -103,6 +107,7 @@ struct LayoutDev {
 
 struct ProfEvent {
     int kind;  // 0 scan, 1 merge, 2 rerank(part of merge), 3 build, 4 plan (host)
+    int pass = 0;  // scans: 0 single or main pass, 1 seeded prefix pass, 2 masked rescan
     hipEvent_t a = nullptr, b = nullptr;
     double host_ms = 0.0;  // kind 4: wall time on the host
 };
@@ -118,7 +123,11 @@ struct Scratch {
     hipEvent_t ev_stage[2] = {nullptr, nullptr};  // recorded after the copies out of h_stage[i]
     int stage_next = 0;
     hipEvent_t ev_nflag = nullptr;  // recorded after the rescan count is copied to h_nflag
-    hipStream_t last_stream = nullptr;
+    // Recorded on the search's stream after the last device work of every
+    // search that uses this scratch; the next search (on any stream) makes its
+    // stream wait for it.  An event, not the previous stream's handle: the
+    // caller may destroy that stream between searches.
+    hipEvent_t ev_done = nullptr;
     bool used = false;
     void release() {
         for (DevBuf *b : {&d_items, &d_iq, &d_slot_off, &d_lists, &d_counts, &d_queue, &d_q, &d_out_ids,
@@ -133,6 +142,9 @@ struct Scratch {
         h_nflag.release();
         if (ev_nflag) (void)hipEventDestroy(ev_nflag);
         ev_nflag = nullptr;
+        if (ev_done) (void)hipEventDestroy(ev_done);
+        ev_done = nullptr;
+        used = false;
     }
 };
 
@@ -159,6 +171,7 @@ struct fbf_dataset {
     bool profile = false;
     std::vector<ProfEvent> events;
     uint64_t last_pairs = 0, last_pairs_computed = 0, last_items = 0;
+    uint64_t last_pairs_prefix = 0;  // pairs of the seeded prefix pass (0: none)
     bool last_seeded = false;  // the last search ran the seeded prefix pass (h_nflag is live)
     uint64_t last_fallbacks = 0;  // FBF_MODE_EXACT_BF16: queries searched again by the FP32 scan
 };
@@ -784,12 +797,23 @@ static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, 
     std::lock_guard<std::mutex> lock(ds->mu);
     HIP_TRY(hipSetDevice(ds->device));
     const auto t_plan0 = std::chrono::steady_clock::now();
-    // The dataset's scratch is reused by every search: a search on another
-    // stream than the previous one first waits for that one.
+    // The dataset's scratch is reused by every search: this search's stream
+    // first waits for the device work of the previous one (whatever stream
+    // that ran on), and this search records the same event when its own work
+    // is queued (on every return path below, failures included).
+    struct DoneRecord {
+        hipEvent_t ev = nullptr;
+        hipStream_t st = nullptr;
+        ~DoneRecord() {
+            if (ev) (void)hipEventRecord(ev, st);
+        }
+    } done_rec;
     if (shared_scratch) {
-        if (S.used && S.last_stream != st) HIP_TRY(hipStreamSynchronize(S.last_stream));
+        if (!S.ev_done) HIP_TRY(hipEventCreateWithFlags(&S.ev_done, hipEventDisableTiming));
+        else if (S.used) HIP_TRY(hipStreamWaitEvent(st, S.ev_done, 0));
         S.used = true;
-        S.last_stream = st;
+        done_rec.ev = S.ev_done;
+        done_rec.st = st;
     }
     // Profiling events and the rescan count are host-side records of a run:
     // neither goes into a captured graph.
@@ -842,11 +866,13 @@ static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, 
             order(P);
             tb.join();
             n_items_a = P.items.size();
+            ds->last_pairs_prefix = P.pairs;
             s = concat_plans(P, PB, nq);
             if (s != FBF_OK) return s;
         }
     }
     if (!seeded) {
+        ds->last_pairs_prefix = 0;
         make_plan(qr, chunk, gmax, P);
         order(P);
         n_items_a = 0;
@@ -972,6 +998,7 @@ static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, 
                 HIP_TRY(launch_fill_u32(queue, 0u, 8 * 16, st));
             }
             ProfEvent ev;
+            ev.pass = pre ? 1 : mask ? 2 : 0;
             if (prof) record_begin(ds, 0, st, ev);
             HIP_TRY(launch_scan_mfma(variant, a, pre ? B_pre : B, nw, st));
             if (prof) record_end(ds, st, ev);
@@ -1110,10 +1137,11 @@ static fbf_status search_exact_bf16(fbf_dataset *ds, const float *queries_dev, c
     if (s != FBF_OK) return s;
     // The search statistics stay those of the BF16 search (the fallback is
     // counted by last_fallbacks).
-    uint64_t pairs = 0, pairs_c = 0, items = 0;
+    uint64_t pairs = 0, pairs_c = 0, items = 0, pairs_pre = 0;
     bool seeded = false;
     {
         std::lock_guard<std::mutex> lock(ds->mu);
+        pairs_pre = ds->last_pairs_prefix;
         pairs = ds->last_pairs;
         pairs_c = ds->last_pairs_computed;
         items = ds->last_items;
@@ -1147,6 +1175,7 @@ static fbf_status search_exact_bf16(fbf_dataset *ds, const float *queries_dev, c
     }
     std::lock_guard<std::mutex> lock(ds->mu);
     ds->last_fallbacks = sub.size();
+    ds->last_pairs_prefix = pairs_pre;
     ds->last_pairs = pairs;
     ds->last_pairs_computed = pairs_c;
     ds->last_items = items;
@@ -1280,6 +1309,43 @@ extern "C" fbf_status fbf_dataset_quant_params(const fbf_dataset *ds, float *alp
     return FBF_OK;
 }
 
+// Rows of one sorted layout as stored on the device: original ids and the
+// quantiser codes (symmetric int8 rows of 112 bytes, or the per-dimension
+// uint8 codes unpacked from their MFMA fragment order).
+extern "C" fbf_status fbf_dataset_layout_rows(const fbf_dataset *ds, uint32_t layout, uint64_t row0, uint64_t nrows,
+                                              uint32_t repr, uint32_t *ids, void *codes) {
+    if (!ds) return fail(FBF_ERR_INVALID_ARG, "null dataset");
+    if (layout > 1) return fail(FBF_ERR_INVALID_ARG, "layout must be 0 or 1");
+    if (row0 > ds->n || nrows > ds->n - row0) return fail(FBF_ERR_INVALID_ARG, "rows outside the layout");
+    if (codes && repr != FBF_REPR_SQ8SYM && repr != FBF_REPR_SQ8)
+        return fail(FBF_ERR_INVALID_ARG, "codes are kept for FBF_REPR_SQ8SYM and FBF_REPR_SQ8 only");
+    if (codes && !(ds->repr & repr)) return fail(FBF_ERR_NO_REPR, "dataset was created without these codes");
+    if (nrows == 0) return FBF_OK;
+    HIP_TRY(hipSetDevice(ds->device));
+    const LayoutDev &D = ds->lay[layout];
+    if (ids) HIP_TRY(hipMemcpy(ids, D.ids + row0, nrows * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (!codes) return FBF_OK;
+    if (repr == FBF_REPR_SQ8SYM) {
+        HIP_TRY(hipMemcpy(codes, D.sym + row0 * 112, nrows * 112, hipMemcpyDeviceToHost));
+        return FBF_OK;
+    }
+    // uint8 codes: 32-row blocks of 7 steps x 64 lanes x 8 bytes; lane l of
+    // step s holds row l % 32 of the block at dims 16 s + 8 (l / 32) + e.
+    constexpr uint64_t kBlockBytes = 7 * 64 * 8;
+    const uint64_t b0 = row0 / 32, b1 = (row0 + nrows + 31) / 32;
+    std::vector<uint8_t> blk((b1 - b0) * kBlockBytes);
+    HIP_TRY(hipMemcpy(blk.data(), (const uint8_t *)D.sq8 + b0 * kBlockBytes, blk.size(), hipMemcpyDeviceToHost));
+    uint8_t *out = static_cast<uint8_t *>(codes);
+    for (uint64_t r = 0; r < nrows; ++r) {
+        const uint64_t p = row0 + r, b = p / 32 - b0, row = p % 32;
+        for (int k = 0; k < kDim; ++k) {
+            const int st = k / 16, half = (k % 16) / 8, e = k % 8;
+            out[r * kDim + k] = blk[b * kBlockBytes + ((uint64_t)st * 64 + half * 32 + row) * 8 + e];
+        }
+    }
+    return FBF_OK;
+}
+
 // ---------------------------------------------------------------- measurement
 extern "C" fbf_status fbf_profile_enable(fbf_dataset *ds, int enable) {
     if (!ds) return fail(FBF_ERR_INVALID_ARG, "null dataset");
@@ -1291,7 +1357,11 @@ extern "C" fbf_status fbf_profile_collect(fbf_dataset *ds, const char *kernel, u
                                           double *total_ms) {
     if (!ds || !kernel) return fail(FBF_ERR_INVALID_ARG, "null argument");
     int want = -1;
+    int want_pass = -1;
     if (!std::strcmp(kernel, "scan")) want = 0;
+    else if (!std::strcmp(kernel, "scan_main")) want = 0, want_pass = 0;
+    else if (!std::strcmp(kernel, "scan_prefix")) want = 0, want_pass = 1;
+    else if (!std::strcmp(kernel, "scan_rescan")) want = 0, want_pass = 2;
     else if (!std::strcmp(kernel, "merge")) want = 1;
     else if (!std::strcmp(kernel, "rerank")) want = 2;
     else if (!std::strcmp(kernel, "build")) want = 3;
@@ -1301,7 +1371,7 @@ extern "C" fbf_status fbf_profile_collect(fbf_dataset *ds, const char *kernel, u
     double ms = 0.0;
     std::vector<ProfEvent> keep;
     for (auto &ev : ds->events) {
-        if (want >= 0 && ev.kind != want) {
+        if ((want >= 0 && ev.kind != want) || (want_pass >= 0 && ev.pass != want_pass)) {
             keep.push_back(ev);
             continue;
         }
@@ -1330,6 +1400,14 @@ extern "C" fbf_status fbf_last_search_stats(const fbf_dataset *ds, uint64_t *pai
     if (pairs) *pairs = ds->last_pairs;
     if (pairs_computed) *pairs_computed = ds->last_pairs_computed;
     if (work_items) *work_items = ds->last_items;
+    return FBF_OK;
+}
+
+extern "C" fbf_status fbf_last_search_pass_pairs(const fbf_dataset *ds, uint64_t *prefix_pairs,
+                                                 uint64_t *main_pairs) {
+    if (!ds) return fail(FBF_ERR_INVALID_ARG, "null dataset");
+    if (prefix_pairs) *prefix_pairs = ds->last_pairs_prefix;
+    if (main_pairs) *main_pairs = ds->last_pairs - ds->last_pairs_prefix;
     return FBF_OK;
 }
 
@@ -1364,7 +1442,9 @@ struct CodeReg {
     int8_t *dev;
 };
 static std::mutex g_reg_mu;
-static std::map<uintptr_t, CodeReg> g_reg;
+// Never destroyed (a heap object): its device copies live until they are
+// unregistered or the process ends, and no destructor runs at exit.
+static std::map<uintptr_t, CodeReg> &g_reg = *new std::map<uintptr_t, CodeReg>();
 
 extern "C" fbf_status fbf_register_codes(const void *host_base, uint64_t bytes) {
     if (!host_base || bytes == 0) return fail(FBF_ERR_INVALID_ARG, "null or empty code array");
@@ -1409,29 +1489,75 @@ static const int8_t *resolve_codes(const void *p, uint64_t bytes) {
     return it->second.dev + (a - base);
 }
 
-// Calling thread's engine for dataset-free calls: own stream and scratch, so
-// the router's OpenMP threads do not serialise on one another.
+// Engine of the dataset-free calls: a stream and scratch of its own, so the
+// router's OpenMP threads do not serialise on one another.  A thread borrows
+// an engine from a process-wide pool on its first call and hands it back when
+// the thread ends; engines are never freed.  Nothing here calls HIP from a
+// destructor: at process exit the HIP runtime may already be tearing down
+// (thread and static destructors run after it has begun to), and the device
+// memory of the pool goes with the process.
 struct ThreadEngine {
+    int device = -1;
     hipStream_t st = nullptr;
     Scratch scr;
     DevBuf tmp_codes;
     HostPinned h_out;
-    ~ThreadEngine() {
-        if (st) (void)hipStreamDestroy(st);
-        scr.release();
-        tmp_codes.release();
-        h_out.release();
+};
+static std::mutex g_eng_mu;
+static std::vector<ThreadEngine *> *g_eng_free = new std::vector<ThreadEngine *>();  // never destroyed
+
+struct EngineLease {
+    ThreadEngine *e = nullptr;
+    ~EngineLease() {
+        if (!e) return;
+        std::lock_guard<std::mutex> lk(g_eng_mu);
+        g_eng_free->push_back(e);  // host bookkeeping only
     }
 };
-static thread_local ThreadEngine t_eng;
+static thread_local EngineLease t_lease;
+
+// The calling thread's engine on the current device (stream created on first use).
+static fbf_status thread_engine(ThreadEngine *&out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (t_lease.e && t_lease.e->device != dev) {
+        std::lock_guard<std::mutex> lk(g_eng_mu);
+        g_eng_free->push_back(t_lease.e);
+        t_lease.e = nullptr;
+    }
+    if (!t_lease.e) {
+        std::lock_guard<std::mutex> lk(g_eng_mu);
+        for (size_t i = 0; i < g_eng_free->size(); ++i)
+            if ((*g_eng_free)[i]->device == dev) {
+                t_lease.e = (*g_eng_free)[i];
+                g_eng_free->erase(g_eng_free->begin() + (long)i);
+                break;
+            }
+    }
+    if (!t_lease.e) {
+        auto *e = new (std::nothrow) ThreadEngine();
+        if (!e) return fail(FBF_ERR_DEVICE, "out of host memory");
+        e->device = dev;
+        const hipError_t he = hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking);
+        if (he != hipSuccess) {
+            delete e;
+            return fail(FBF_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(he));
+        }
+        t_lease.e = e;
+    }
+    out = t_lease.e;
+    return FBF_OK;
+}
 
 // Symmetric int8 top-k over nq slices of one device code array.  out_pos /
 // out_dist are host arrays [nq][k]; positions are relative to each slice.
 static fbf_status sym_topk(const int8_t *codes_dev, uint64_t n_rows, const uint64_t *starts,
                            const uint64_t *counts, const int8_t *qcodes_host, uint64_t nq, uint32_t k,
                            int32_t *out_pos, int32_t *out_dist) {
-    ThreadEngine &E = t_eng;
-    if (!E.st) HIP_TRY(hipStreamCreateWithFlags(&E.st, hipStreamNonBlocking));
+    ThreadEngine *Ep = nullptr;
+    fbf_status se = thread_engine(Ep);
+    if (se != FBF_OK) return se;
+    ThreadEngine &E = *Ep;
     hipStream_t st = E.st;
     std::vector<QRange> qr(nq);
     for (uint64_t i = 0; i < nq; ++i) {
@@ -1489,15 +1615,19 @@ static fbf_status bruteforce_common(size_t k, int *closest_points, float *distan
     if (k == 0) return FBF_OK;  // the reference writes nothing either
     if (!closest_points || !queries_in || (npoints && !codes_in))
         return fail(FBF_ERR_INVALID_ARG, "null argument");
-    if (k > 1024) return fail(FBF_ERR_INVALID_ARG, "k must be <= 1024");
+    // A list keeps k keys plus one tile (64) of appends: k <= 896 fits the
+    // 1024-key lists (list_capacity).
+    if (k > 896) return fail(FBF_ERR_INVALID_ARG, "k must be <= 896");
     std::vector<int32_t> pos(k, -1), dist(k, INT32_MAX);
     if (npoints > 0) {
         const uint64_t bytes = (uint64_t)npoints * 112;
         const int8_t *dev = resolve_codes(codes_in, bytes);
         if (!dev) {
             // Not registered: stage this slice (correct, but a copy per call).
-            ThreadEngine &E = t_eng;
-            if (!E.st) HIP_TRY(hipStreamCreateWithFlags(&E.st, hipStreamNonBlocking));
+            ThreadEngine *Ep = nullptr;
+            fbf_status se = thread_engine(Ep);
+            if (se != FBF_OK) return se;
+            ThreadEngine &E = *Ep;
             HIP_TRY(E.tmp_codes.reserve(bytes));
             HIP_TRY(hipMemcpyAsync(E.tmp_codes.p, codes_in, bytes, hipMemcpyHostToDevice, E.st));
             dev = E.tmp_codes.as<int8_t>();
@@ -1533,7 +1663,7 @@ extern "C" fbf_status fbf_bruteforce_batch(const char *codes_base, const uint64_
                                            uint32_t k, int32_t *out_pos, int32_t *out_dist) {
     if (!codes_base || !starts || !counts || !qcodes || !out_pos || !out_dist)
         return fail(FBF_ERR_INVALID_ARG, "null argument");
-    if (k == 0 || k > 1024) return fail(FBF_ERR_INVALID_ARG, "k must be in [1, 1024]");
+    if (k == 0 || k > 896) return fail(FBF_ERR_INVALID_ARG, "k must be in [1, 896]");
     if (nq == 0) return FBF_OK;
     uint64_t rows = 0;
     for (uint64_t i = 0; i < nq; ++i) rows = std::max(rows, starts[i] + counts[i]);

@@ -176,6 +176,8 @@ __global__ __launch_bounds__(64) void scan_f32_kernel(ScanArgs a) {
 
 hipError_t launch_scan_f32(const ScanArgs &a, int B, hipStream_t st) {
     if (a.n_items == 0) return hipSuccess;
+    // A list holds K kept keys plus one tile of appends (offer_global).
+    if ((int)a.K + kRowTile > B) return hipErrorInvalidValue;
     uint64_t grid = a.n_items;
     if (a.queue) {
         int dev = 0, cus = 0;
@@ -366,6 +368,8 @@ hipError_t launch_merge(const MergeArgs &a, hipStream_t st) {
     if (a.nq == 0) return hipSuccess;
     const uint32_t need = a.kprime > a.k ? a.kprime : a.k;
     const int B = list_capacity((int)need);
+    // The LDS list keeps need keys plus one 64-key batch of offers (offer_lds).
+    if ((int)need + 64 > B) return hipErrorInvalidValue;
     switch (B) {
     case 256: hipLaunchKernelGGL(merge_kernel<256>, dim3((uint32_t)a.nq), dim3(64), 0, st, a); break;
     case 512: hipLaunchKernelGGL(merge_kernel<512>, dim3((uint32_t)a.nq), dim3(64), 0, st, a); break;

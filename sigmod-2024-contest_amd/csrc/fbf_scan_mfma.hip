@@ -1004,6 +1004,9 @@ hipError_t launch_scan_mfma(int variant, const ScanMfmaArgs &a, int B, int nw, h
     if (a.n_items == 0) return hipSuccess;
     if (nw != 1 && nw != 2 && nw != 4) return hipErrorInvalidValue;
     if ((uint32_t)B > a.list_stride) return hipErrorInvalidValue;
+    // After a compaction a list half holds ceil(K / 2) keys and must take one
+    // more tile of appends (16 per lane) before the next check.
+    if ((a.K + 1) / 2 + 16 > (uint32_t)B / 2) return hipErrorInvalidValue;
     uint64_t grid = a.n_items;
     if (a.queue) {
         int dev = 0, cus = 0;

@@ -130,6 +130,8 @@ __global__ __launch_bounds__(64) void scan_sym_kernel(ScanSymArgs a) {
 
 hipError_t launch_scan_sq8sym(const ScanSymArgs &a, int B, hipStream_t st) {
     if (a.n_items == 0) return hipSuccess;
+    // A list holds K kept keys plus one tile of appends (offer_global).
+    if ((int)a.K + kRowTile > B) return hipErrorInvalidValue;
     uint64_t grid = a.n_items;
     if (a.queue) {
         int dev = 0, cus = 0;

@@ -16,6 +16,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys as _sys
 
 import numpy as np
 
@@ -39,7 +40,8 @@ EXPORTS = [
     "fbf_unregister_codes", "fbf_bruteforce", "fbf_bruteforce_subgraph", "fbf_bruteforce_batch",
     "fbf_profile_enable", "fbf_profile_collect", "fbf_last_search_stats", "fbf_last_search_rescans",
     "fbf_last_error", "fbf_version", "fbf_search_graph_create", "fbf_search_graph_launch",
-    "fbf_search_graph_destroy", "fbf_last_search_fallbacks",
+    "fbf_search_graph_destroy", "fbf_last_search_fallbacks", "fbf_dataset_layout_rows",
+    "fbf_last_search_pass_pairs",
 ]
 
 
@@ -87,10 +89,12 @@ def lib() -> C.CDLL:
                                         C.POINTER(C.c_uint64)]
     L.fbf_last_search_rescans.argtypes = [_P, C.POINTER(C.c_uint64)]
     L.fbf_last_search_fallbacks.argtypes = [_P, C.POINTER(C.c_uint64)]
+    L.fbf_last_search_pass_pairs.argtypes = [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     L.fbf_search_graph_create.argtypes = [_P, _P, _P, C.c_uint64, C.c_uint32, C.POINTER(SearchOpts), _P, _P,
                                           C.POINTER(_P)]
     L.fbf_search_graph_launch.argtypes = [_P, _P]
     L.fbf_search_graph_destroy.argtypes = [_P]
+    L.fbf_dataset_layout_rows.argtypes = [_P, C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint32, _P, _P]
     for name in EXPORTS:
         if name not in ("fbf_last_error", "fbf_version"):
             getattr(L, name).restype = C.c_int
@@ -147,6 +151,10 @@ class Dataset:
             self.h = None
 
     def __del__(self):
+        # At interpreter shutdown the HIP runtime may already be tearing down:
+        # no device calls then (the process exit releases the memory).
+        if _sys is None or _sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -210,11 +218,32 @@ class Dataset:
         return ids, dists
 
     def quant_params(self):
+        """(alpha, mi, dif): the symmetric int8 scale (None without
+        REPR_SQ8SYM) and the per-dimension uint8 min / range (None without
+        REPR_SQ8)."""
         a = C.c_float(0)
-        mi = np.zeros(self.d, np.float32); dif = np.zeros(self.d, np.float32)
-        _check(lib().fbf_dataset_quant_params(self.h, C.byref(a), _np_ptr(mi), _np_ptr(dif)),
+        has_sym, has_u8 = bool(self.repr_flags & REPR_SQ8SYM), bool(self.repr_flags & REPR_SQ8)
+        mi = np.zeros(self.d, np.float32) if has_u8 else None
+        dif = np.zeros(self.d, np.float32) if has_u8 else None
+        _check(lib().fbf_dataset_quant_params(self.h, C.byref(a) if has_sym else None,
+                                              _np_ptr(mi) if has_u8 else None, _np_ptr(dif) if has_u8 else None),
                "fbf_dataset_quant_params")
-        return a.value, mi, dif
+        return (a.value if has_sym else None), mi, dif
+
+    def layout_rows(self, layout: int, row0: int = 0, nrows: int | None = None, repr_flag: int = 0):
+        """Original ids and (repr_flag REPR_SQ8SYM / REPR_SQ8) the device codes of
+        positions [row0, row0 + nrows) of one sorted layout."""
+        nrows = self.n - row0 if nrows is None else nrows
+        ids = np.zeros(nrows, np.uint32)
+        codes = None
+        if repr_flag == REPR_SQ8SYM:
+            codes = np.zeros((nrows, CODE_BYTES), np.int8)
+        elif repr_flag == REPR_SQ8:
+            codes = np.zeros((nrows, self.d), np.uint8)
+        _check(lib().fbf_dataset_layout_rows(self.h, layout, row0, nrows, repr_flag, _np_ptr(ids),
+                                             None if codes is None else _np_ptr(codes)),
+               "fbf_dataset_layout_rows")
+        return ids, codes
 
     # ---------------------------------------------------------- measurement
     def profile(self, enable: bool = True) -> None:
@@ -231,6 +260,12 @@ class Dataset:
         _check(lib().fbf_last_search_stats(self.h, C.byref(p), C.byref(pc), C.byref(wi)),
                "fbf_last_search_stats")
         return {"pairs": p.value, "pairs_computed": pc.value, "work_items": wi.value}
+
+    def last_pass_pairs(self) -> tuple[int, int]:
+        """(prefix-pass pairs, main-pass pairs) of the last search."""
+        a = C.c_uint64(0); b = C.c_uint64(0)
+        _check(lib().fbf_last_search_pass_pairs(self.h, C.byref(a), C.byref(b)), "fbf_last_search_pass_pairs")
+        return a.value, b.value
 
     def last_rescans(self) -> int:
         """Queries of the last search rescanned after their seed failed."""
@@ -326,6 +361,8 @@ class SearchGraph:
             self.h = None
 
     def __del__(self):
+        if _sys is None or _sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:

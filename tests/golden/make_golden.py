@@ -14,6 +14,10 @@ were made).
   bruteforce() and bruteforce_subgraph() outputs (heap-array order) at
   k = 140 over the whole code array and over a slice.
 * kat.json: the int8 known answer of test_simd_distance.cpp:199-203.
+* ref_router_bf.npz: the reference's brute-force route (hybrid_graph.cpp:
+  203-228: bruteforce() top-140 of the symmetric int8 codes, id map, L2SqrRef
+  rerank to 100 in maxPQIFCS) run by the reference kernels over every dummy
+  query's filtered range of the sorted layouts; ids in heap-array order.
 """
 from __future__ import annotations
 
@@ -75,6 +79,21 @@ def main() -> None:
     np.savez_compressed(os.path.join(HERE, "ref_bruteforce.npz"), codes=codes, alpha=np.float32(alpha),
                         query_index=sel, qcodes=qcodes, k=k, bf_full=bf_full, slice=np.array([s0, s1]),
                         bf_slice=bf_slice, bf_slice_dist=bf_slice_d)
+
+    # The reference's brute-force route over every dummy query's range.
+    L = orc.Layouts(labels.astype(np.int32), ts)
+    ranges = L.ranges(metas).astype(np.int64)
+    codes_b = np.ascontiguousarray(codes[L.perm_both])
+    codes_t = np.ascontiguousarray(codes[L.perm_time])
+    qcodes_all = np.zeros((nq, 112), np.int8)
+    for i in range(nq):
+        R.ref_sq8sym_encode(orc._p(np.ascontiguousarray(q[i])), d, alpha, orc._p(qcodes_all[i]))
+    rr_ids = np.zeros((nq, 100), np.uint32)
+    R.ref_router_bf(orc._p(codes_b), orc._p(codes_t), orc._p(L.perm_both), orc._p(L.perm_time),
+                    orc._p(np.ascontiguousarray(vecs)), d, alpha, orc._p(np.ascontiguousarray(q)),
+                    orc._p(qcodes_all), orc._p(np.ascontiguousarray(ranges)), nq, 140, 100, orc._p(rr_ids))
+    np.savez_compressed(os.path.join(HERE, "ref_router_bf.npz"), ranges=ranges, ids=rr_ids, kq=140, k=100,
+                        alpha=np.float32(alpha))
 
     a0 = [127, 127, 127, -4, 5, 6, 7, 8, 9, -127, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, -24,
           25, 26, 27, 28, 29, 30, 31, 32]

@@ -47,6 +47,17 @@ def test_bruteforce_argument_checks(pkg):
     assert L.fbf_bruteforce(100, 4, out.ctypes.data_as(C.c_void_p), 10, None, 1.0, 1,
                             q.ctypes.data_as(C.c_void_p)) == 1
     assert L.fbf_unregister_codes(C.c_void_p(0x1234)) == 3
+    # k beyond the 1024-key lists' room (k + one tile of appends): rejected
+    # before any device work
+    big = np.zeros(1000, np.int32)
+    codes = np.zeros((10, 112), np.int8)
+    assert L.fbf_bruteforce(100, 1000, big.ctypes.data_as(C.c_void_p), 10, codes.ctypes.data_as(C.c_void_p),
+                            1.0, 1, q.ctypes.data_as(C.c_void_p)) == 1
+    assert b"896" in L.fbf_last_error()
+    st = np.zeros(1, np.uint64); cn = np.ones(1, np.uint64)
+    assert L.fbf_bruteforce_batch(codes.ctypes.data_as(C.c_void_p), st.ctypes.data_as(C.c_void_p),
+                                  cn.ctypes.data_as(C.c_void_p), q.ctypes.data_as(C.c_void_p), 1, 897,
+                                  big.ctypes.data_as(C.c_void_p), big.ctypes.data_as(C.c_void_p)) == 1
 
 
 def test_shim_header_compiles_against_abi(tmp_path):
