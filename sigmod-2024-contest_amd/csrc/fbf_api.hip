@@ -1549,8 +1549,87 @@ static fbf_status thread_engine(ThreadEngine *&out) {
     return FBF_OK;
 }
 
-// Symmetric int8 top-k over nq slices of one device code array.  out_pos /
-// out_dist are host arrays [nq][k]; positions are relative to each slice.
+// Device buffers of one symmetric-int8 batch: the plan, the per-(query,
+// chunk) candidate lists, the query codes and the results.
+struct SymBufs {
+    WorkItem *items = nullptr;
+    ItemQuery *iq = nullptr;
+    uint64_t *slot_off = nullptr;
+    uint64_t *lists = nullptr;
+    uint32_t *counts = nullptr;
+    int8_t *qcodes = nullptr;
+    uint32_t *out_ids = nullptr;
+    float *out_d = nullptr;  // raw int distances (merge key_int)
+};
+
+// Plan of nq slices [starts[i], starts[i] + counts[i]) of one code array: up
+// to 16 queries per work item, and a chunk size that gives a batch about 2048
+// work items (4 Ki to 64 Ki rows each), so one call fills the device even for
+// a single long slice.
+static fbf_status sym_plan(const uint64_t *starts, const uint64_t *counts, uint64_t nq, uint64_t n_rows, Plan &P) {
+    std::vector<QRange> qr(nq);
+    uint64_t total = 0;
+    for (uint64_t i = 0; i < nq; ++i) {
+        if (starts[i] + counts[i] > n_rows || starts[i] + counts[i] >= 0xFFFFFFFFull)
+            return fail(FBF_ERR_INVALID_ARG, "slice outside the code array");
+        qr[i] = QRange{0, (uint32_t)starts[i], (uint32_t)(starts[i] + counts[i]), (uint32_t)i};
+        total += counts[i];
+    }
+    const uint64_t c = (total / 2048 + kRowTile - 1) / kRowTile * kRowTile;
+    const uint32_t chunk = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(c, 4096), 65536);
+    make_plan(qr, chunk, kMaxGroup, P);
+    return finish_slots(P, nq);
+}
+
+// Queue the scan and the merge of a planned batch whose plan and query codes
+// are in b (on st).
+static fbf_status sym_launch(const int8_t *codes_dev, uint64_t n_rows, const Plan &P, uint64_t nq, uint32_t k,
+                             int B, const SymBufs &b, hipStream_t st) {
+    ScanSymArgs a{};
+    a.items = b.items;
+    a.n_items = P.items.size();
+    a.iq = b.iq;
+    a.qcodes = b.qcodes;
+    a.codes = codes_dev;
+    a.ids = nullptr;
+    a.n_rows = n_rows;
+    a.lists = b.lists;
+    a.counts = b.counts;
+    a.K = k;
+    a.queue = nullptr;
+    HIP_TRY(launch_scan_sq8sym(a, B, st));
+    MergeArgs m{};
+    m.slot_off = b.slot_off;
+    m.lists = b.lists;
+    m.counts = b.counts;
+    m.list_cap = (uint32_t)B;
+    m.nq = nq;
+    m.k = k;
+    m.kprime = k;
+    m.out_ids = b.out_ids;
+    m.out_dists = b.out_d;
+    m.rerank = 0;
+    m.key_int = 1;
+    HIP_TRY(launch_merge(m, st));
+    return FBF_OK;
+}
+
+// Results of the merge (absolute row ids, raw int distances) as slice-relative
+// positions and int distances; pads are -1 / INT32_MAX.  ids may alias out_pos
+// and dbits out_dist.
+static void sym_unpack(const uint32_t *ids, const uint32_t *dbits, const uint64_t *starts, uint64_t nq, uint32_t k,
+                       int32_t *out_pos, int32_t *out_dist) {
+    for (uint64_t i = 0; i < nq * (uint64_t)k; ++i) {
+        const uint32_t id = ids[i];
+        const uint32_t db = dbits[i];
+        out_pos[i] = id == kPadId ? -1 : (int32_t)(id - (uint32_t)starts[i / k]);
+        out_dist[i] = id == kPadId ? INT32_MAX : (int32_t)db;
+    }
+}
+
+// Symmetric int8 top-k over nq slices of one device code array, on the
+// calling thread's engine.  out_pos / out_dist are host arrays [nq][k];
+// positions are relative to each slice.
 static fbf_status sym_topk(const int8_t *codes_dev, uint64_t n_rows, const uint64_t *starts,
                            const uint64_t *counts, const int8_t *qcodes_host, uint64_t nq, uint32_t k,
                            int32_t *out_pos, int32_t *out_dist) {
@@ -1559,54 +1638,34 @@ static fbf_status sym_topk(const int8_t *codes_dev, uint64_t n_rows, const uint6
     if (se != FBF_OK) return se;
     ThreadEngine &E = *Ep;
     hipStream_t st = E.st;
-    std::vector<QRange> qr(nq);
-    for (uint64_t i = 0; i < nq; ++i) {
-        if (starts[i] + counts[i] > n_rows || starts[i] + counts[i] >= 0xFFFFFFFFull)
-            return fail(FBF_ERR_INVALID_ARG, "slice outside the code array");
-        qr[i] = QRange{0, (uint32_t)starts[i], (uint32_t)(starts[i] + counts[i]), (uint32_t)i};
-    }
     Plan P;
-    make_plan(qr, 64 * 1024, kMaxGroup, P);
-    fbf_status s0 = finish_slots(P, nq);
-    if (s0 != FBF_OK) return s0;
+    fbf_status s = sym_plan(starts, counts, nq, n_rows, P);
+    if (s != FBF_OK) return s;
     const int B = list_capacity((int)k);
     Scratch &S = E.scr;
-    fbf_status s = upload_plan(S, P, nq, B, st);
+    s = upload_plan(S, P, nq, B, st);
     if (s != FBF_OK) return s;
     HIP_TRY(S.d_q.reserve(nq * 112));
     HIP_TRY(S.d_out_ids.reserve(nq * (size_t)k * 4));
     HIP_TRY(S.d_out_d.reserve(nq * (size_t)k * 4));
     HIP_TRY(hipMemcpyAsync(S.d_q.p, qcodes_host, nq * 112, hipMemcpyHostToDevice, st));
-    ScanSymArgs a{};
-    a.items = S.d_items.as<WorkItem>();
-    a.n_items = P.items.size();
-    a.iq = S.d_iq.as<ItemQuery>();
-    a.qcodes = S.d_q.as<int8_t>();
-    a.codes = codes_dev;
-    a.ids = nullptr;
-    a.n_rows = n_rows;
-    a.lists = S.d_lists.as<uint64_t>();
-    a.counts = S.d_counts.as<uint32_t>();
-    a.K = k;
-    a.queue = nullptr;
-    HIP_TRY(launch_scan_sq8sym(a, B, st));
-    MergeArgs m = merge_args(S, B, nq, k, k);
-    m.out_ids = S.d_out_ids.as<uint32_t>();
-    m.out_dists = S.d_out_d.as<float>();
-    m.rerank = 0;
-    m.key_int = 1;
-    HIP_TRY(launch_merge(m, st));
+    SymBufs b;
+    b.items = S.d_items.as<WorkItem>();
+    b.iq = S.d_iq.as<ItemQuery>();
+    b.slot_off = S.d_slot_off.as<uint64_t>();
+    b.lists = S.d_lists.as<uint64_t>();
+    b.counts = S.d_counts.as<uint32_t>();
+    b.qcodes = S.d_q.as<int8_t>();
+    b.out_ids = S.d_out_ids.as<uint32_t>();
+    b.out_d = S.d_out_d.as<float>();
+    s = sym_launch(codes_dev, n_rows, P, nq, k, B, b, st);
+    if (s != FBF_OK) return s;
     HIP_TRY(E.h_out.reserve(nq * (size_t)k * 8));
     uint32_t *hp = (uint32_t *)E.h_out.p;
     HIP_TRY(hipMemcpyAsync(hp, S.d_out_ids.p, nq * (size_t)k * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(hp + nq * (size_t)k, S.d_out_d.p, nq * (size_t)k * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
-    for (uint64_t i = 0; i < nq * (uint64_t)k; ++i) {
-        const uint32_t id = hp[i];
-        // Slice-relative position; the row id in the key is absolute.
-        out_pos[i] = id == kPadId ? -1 : (int32_t)(id - (uint32_t)starts[i / k]);
-        out_dist[i] = id == kPadId ? INT32_MAX : (int32_t)hp[nq * (size_t)k + i];
-    }
+    sym_unpack(hp, hp + nq * (size_t)k, starts, nq, k, out_pos, out_dist);
     return FBF_OK;
 }
 
