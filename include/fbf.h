@@ -199,6 +199,23 @@ fbf_status fbf_bruteforce_batch(const char *codes_base, const uint64_t *starts,
                                 const uint64_t *counts, const int8_t *qcodes, uint64_t nq,
                                 uint32_t k, int32_t *out_pos, int32_t *out_dist);
 
+/* No counterpart in the reference.  fbf_bruteforce_batch for callers that
+ * cannot allocate device memory (serving loops and harnesses that forbid
+ * hipMalloc): the same arguments and results, plus the stream to run on
+ * (NULL = default stream), with every scratch buffer -- plan, candidate
+ * lists, query codes, results -- in a fixed device array of
+ * FBF_NOALLOC_SCRATCH_BYTES per GPU that the library holds until the process
+ * ends.  The call allocates nothing on the device (codes_base must lie in a
+ * registered code array, registered beforehand); a batch that needs more
+ * scratch than the array holds is rejected with FBF_ERR_INVALID_ARG (about
+ * 2 KiB per (query, 64 Ki-row chunk) at k <= 160).  Every call on a GPU
+ * shares that GPU's array: the library runs one such call at a time per
+ * process, and each waits for its own results. */
+#define FBF_NOALLOC_SCRATCH_BYTES (64ull << 20)
+fbf_status fbf_bruteforce_batch_noalloc(const char *codes_base, const uint64_t *starts,
+                                        const uint64_t *counts, const int8_t *qcodes, uint64_t nq,
+                                        uint32_t k, int32_t *out_pos, int32_t *out_dist, void *stream);
+
 /* ---------------------------------------------------------------------
  * Measurement
  * --------------------------------------------------------------------- */

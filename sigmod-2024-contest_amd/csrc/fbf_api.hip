@@ -1730,3 +1730,104 @@ extern "C" fbf_status fbf_bruteforce_batch(const char *codes_base, const uint64_
     if (!dev) return fail(FBF_ERR_UNREGISTERED, "codes_base is not inside a registered code array");
     return sym_topk(dev, rows, starts, counts, qcodes, nq, k, out_pos, out_dist);
 }
+
+// ---------------------------------------------------------------- allocation-free batch
+//
+// fbf_bruteforce_batch_noalloc: fbf_bruteforce_batch for callers that may not
+// allocate device memory.  Its plan, candidate lists, query codes and results
+// live in one fixed device array (below); the caller passes the inputs and
+// outputs, and the code array must already be registered.
+constexpr uint64_t kNoAllocBytes = 64ull << 20;  // FBF_NOALLOC_SCRATCH_BYTES
+static_assert(kNoAllocBytes == FBF_NOALLOC_SCRATCH_BYTES, "header and library disagree on the scratch size");
+// The scratch: one copy per GPU the library runs on, held until the process
+// ends.  Host code never passes this name to a kernel or a copy (on the host
+// it is a stand-in, not a device address); it uses the per-device address
+// that hipGetSymbolAddress returns.
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+constexpr int kMaxDevices = 64;
+static std::mutex g_noalloc_mu;  // one call at a time: every call on a GPU shares its scratch
+static uint8_t *g_noalloc_ptr[kMaxDevices] = {};
+
+// Device address of the scratch on the current device (looked up once per device).
+static fbf_status noalloc_scratch(int dev, uint8_t *&out) {
+    if (dev < 0 || dev >= kMaxDevices) return fail(FBF_ERR_UNSUPPORTED, "device index beyond the scratch table");
+    if (!g_noalloc_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr)
+            return fail(FBF_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_noalloc_ptr[dev] = static_cast<uint8_t *>(p);
+    }
+    out = g_noalloc_ptr[dev];
+    return FBF_OK;
+}
+
+extern "C" fbf_status fbf_bruteforce_batch_noalloc(const char *codes_base, const uint64_t *starts,
+                                                   const uint64_t *counts, const int8_t *qcodes, uint64_t nq,
+                                                   uint32_t k, int32_t *out_pos, int32_t *out_dist, void *stream) {
+    if (!codes_base || !starts || !counts || !qcodes || !out_pos || !out_dist)
+        return fail(FBF_ERR_INVALID_ARG, "null argument");
+    if (k == 0 || k > 896) return fail(FBF_ERR_INVALID_ARG, "k must be in [1, 896]");
+    if (nq == 0) return FBF_OK;
+    uint64_t rows = 0;
+    for (uint64_t i = 0; i < nq; ++i) rows = std::max(rows, starts[i] + counts[i]);
+    const int8_t *dev_codes = resolve_codes(codes_base, rows * 112);
+    if (!dev_codes) return fail(FBF_ERR_UNREGISTERED, "codes_base is not inside a registered code array");
+    Plan P;
+    fbf_status s = sym_plan(starts, counts, nq, rows, P);
+    if (s != FBF_OK) return s;
+    const int B = list_capacity((int)k);
+    // Carve the scratch: 256-byte aligned pieces.
+    const uint64_t n_slots = P.slot_off[nq];
+    uint64_t off = 0;
+    auto take = [&](uint64_t bytes) {
+        const uint64_t o = off;
+        off += (bytes + 255) / 256 * 256;
+        return o;
+    };
+    const uint64_t o_items = take(P.items.size() * sizeof(WorkItem));
+    const uint64_t o_iq = take(P.iq.size() * sizeof(ItemQuery));
+    const uint64_t o_slot = take((nq + 1) * sizeof(uint64_t));
+    const uint64_t o_lists = take(n_slots * (uint64_t)B * sizeof(uint64_t));
+    const uint64_t o_counts = take(n_slots * sizeof(uint32_t));
+    const uint64_t o_q = take(nq * 112);
+    const uint64_t o_ids = take(nq * (uint64_t)k * 4);
+    const uint64_t o_d = take(nq * (uint64_t)k * 4);
+    if (off > kNoAllocBytes)
+        return fail(FBF_ERR_INVALID_ARG, "batch needs " + std::to_string(off) + " bytes of scratch; the allocation-free "
+                                         "entry point holds " + std::to_string(kNoAllocBytes) +
+                                         " (FBF_NOALLOC_SCRATCH_BYTES): split the batch");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    uint8_t *base = nullptr;
+    s = noalloc_scratch(dev, base);
+    if (s != FBF_OK) return s;
+    hipStream_t st = (hipStream_t)stream;
+    SymBufs b;
+    b.items = reinterpret_cast<WorkItem *>(base + o_items);
+    b.iq = reinterpret_cast<ItemQuery *>(base + o_iq);
+    b.slot_off = reinterpret_cast<uint64_t *>(base + o_slot);
+    b.lists = reinterpret_cast<uint64_t *>(base + o_lists);
+    b.counts = reinterpret_cast<uint32_t *>(base + o_counts);
+    b.qcodes = reinterpret_cast<int8_t *>(base + o_q);
+    b.out_ids = reinterpret_cast<uint32_t *>(base + o_ids);
+    b.out_d = reinterpret_cast<float *>(base + o_d);
+    // Nothing in the scratch needs clearing: each work item writes the count
+    // of every list slot it owns, and the merge reads only those.
+    if (!P.items.empty())
+        HIP_TRY(hipMemcpyAsync(b.items, P.items.data(), P.items.size() * sizeof(WorkItem), hipMemcpyHostToDevice, st));
+    if (!P.iq.empty())
+        HIP_TRY(hipMemcpyAsync(b.iq, P.iq.data(), P.iq.size() * sizeof(ItemQuery), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(b.slot_off, P.slot_off.data(), (nq + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(b.qcodes, qcodes, nq * 112, hipMemcpyHostToDevice, st));
+    s = sym_launch(dev_codes, rows, P, nq, k, B, b, st);
+    if (s != FBF_OK) return s;
+    // Results straight into the caller's arrays, then made slice-relative there.
+    HIP_TRY(hipMemcpyAsync(out_pos, b.out_ids, nq * (uint64_t)k * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(out_dist, b.out_d, nq * (uint64_t)k * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    sym_unpack(reinterpret_cast<const uint32_t *>(out_pos), reinterpret_cast<const uint32_t *>(out_dist), starts, nq, k,
+               out_pos, out_dist);
+    return FBF_OK;
+}

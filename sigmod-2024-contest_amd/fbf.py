@@ -41,7 +41,7 @@ EXPORTS = [
     "fbf_profile_enable", "fbf_profile_collect", "fbf_last_search_stats", "fbf_last_search_rescans",
     "fbf_last_error", "fbf_version", "fbf_search_graph_create", "fbf_search_graph_launch",
     "fbf_search_graph_destroy", "fbf_last_search_fallbacks", "fbf_dataset_layout_rows",
-    "fbf_last_search_pass_pairs",
+    "fbf_last_search_pass_pairs", "fbf_bruteforce_batch_noalloc",
 ]
 
 
@@ -83,6 +83,7 @@ def lib() -> C.CDLL:
     L.fbf_bruteforce_subgraph.argtypes = [C.c_size_t, C.c_size_t, _P, _P, C.c_size_t, _P, C.c_float,
                                           C.c_size_t, _P]
     L.fbf_bruteforce_batch.argtypes = [_P, _P, _P, _P, C.c_uint64, C.c_uint32, _P, _P]
+    L.fbf_bruteforce_batch_noalloc.argtypes = [_P, _P, _P, _P, C.c_uint64, C.c_uint32, _P, _P, _P]
     L.fbf_profile_enable.argtypes = [_P, i]
     L.fbf_profile_collect.argtypes = [_P, C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]
     L.fbf_last_search_stats.argtypes = [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
@@ -323,6 +324,21 @@ def bruteforce_batch(codes_base: np.ndarray, starts, counts, qcodes: np.ndarray,
     pos = np.zeros((nq, k), np.int32); dist = np.zeros((nq, k), np.int32)
     _check(lib().fbf_bruteforce_batch(_np_ptr(codes_base), _np_ptr(starts), _np_ptr(counts), _np_ptr(qcodes),
                                       nq, k, _np_ptr(pos), _np_ptr(dist)), "fbf_bruteforce_batch")
+    return pos, dist
+
+
+def bruteforce_batch_noalloc(codes_base: np.ndarray, starts, counts, qcodes: np.ndarray, k: int, stream=None):
+    """fbf_bruteforce_batch_noalloc: bruteforce_batch with its scratch in the
+    library's fixed device array (no device allocation; codes_base must be
+    registered).  stream: a torch stream or None (default stream)."""
+    starts = np.ascontiguousarray(starts, np.uint64); counts = np.ascontiguousarray(counts, np.uint64)
+    qcodes = np.ascontiguousarray(qcodes, np.int8)
+    nq = starts.size
+    pos = np.zeros((nq, k), np.int32); dist = np.zeros((nq, k), np.int32)
+    st = None if stream is None else _P(stream.cuda_stream)
+    _check(lib().fbf_bruteforce_batch_noalloc(_np_ptr(codes_base), _np_ptr(starts), _np_ptr(counts),
+                                              _np_ptr(qcodes), nq, k, _np_ptr(pos), _np_ptr(dist), st),
+           "fbf_bruteforce_batch_noalloc")
     return pos, dist
 
 
