@@ -58,6 +58,9 @@ def parse():
     p.add_argument("--graph", action=argparse.BooleanOptionalAction, default=None,
                    help="also time replays of the batch captured as one HIP graph (fbf_search_graph_*); "
                         "default: on for the approximate modes")
+    p.add_argument("--dropin", action="store_true",
+                   help="only time the reference-shaped bruteforce() drop-in on the router-routed slices "
+                        "(selectivity below bf_thr / bf_thr3) beside the CPU bruteforce(), and exit")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                    help="only run a plain device copy of MIB MiB (HBM counter calibration) and exit")
     return p.parse_args()
@@ -94,6 +97,8 @@ def main() -> int:
     vecs, labels, ts = dg.make_base(cfg, n=n)
     q, metas = dg.make_queries(cfg, nq=nq)
     log(f"data ready in {time.time() - t0:.1f}s")
+    if a.dropin:
+        return dropin(a, ge, fbf, cfg, vecs, labels, ts, q, metas)
 
     mode = a.mode
     repr_flags = fbf.MODE_REPR[fbf.MODES[mode]]
@@ -350,6 +355,76 @@ def main() -> int:
     }
     print(json.dumps(line), flush=True)
     ds.close()
+    return 0
+
+
+def dropin(a, ge, fbf, cfg, vecs, labels, ts, q, metas) -> int:
+    """The router's per-query brute-force calls (hybrid_graph.cpp:200-211):
+    queries with selectivity below bf_thr = 0.045 (type 3: bf_thr3 = 0.08)
+    call bruteforce(dim, min(140, e - s), ..., codes + s * 112, ...) on their
+    range of the symmetric int8 codes.  Times, on the same slices and query
+    codes: the GPU drop-in fbf_bruteforce called from 16 host threads, the
+    batched fbf_bruteforce_batch (one call for all of them), and the CPU
+    bruteforce() -- the reference's own kernel (oracle/_ref) when this host
+    can run it, else the oracle's heap restatement -- from 16 host threads."""
+    from concurrent.futures import ThreadPoolExecutor
+    orc = ge.load_oracle()
+    lay = orc.Layouts(labels.astype(np.int32), ts)
+    alpha = orc.sq8sym_train(vecs)
+    codes = orc.sq8sym_encode(vecs, alpha)
+    cb = np.ascontiguousarray(codes[lay.perm_both]); ct = np.ascontiguousarray(codes[lay.perm_time])
+    del codes
+    rr = lay.ranges(metas)
+    cnt = rr[:, 2] - rr[:, 1]
+    sel = cnt / float(vecs.shape[0])
+    routed = np.nonzero((sel < 0.045) | ((metas[:, 0] == 3) & (sel < 0.08)))[0]
+    routed = routed[cnt[routed] > 0][: a.nq or 20000]
+    qc = orc.sq8sym_encode(q[routed], alpha)
+    log(f"drop-in: {routed.size} routed queries, slices of {int(cnt[routed].min())}-{int(cnt[routed].max())} rows "
+        f"(mean {cnt[routed].mean():.0f})")
+    fbf.register_codes(cb); fbf.register_codes(ct)
+    R = orc.ref()
+
+    def gpu_one(j):
+        i = routed[j]
+        arr = cb if rr[i, 0] == 0 else ct
+        return fbf.bruteforce(int(min(140, cnt[i])), int(cnt[i]), arr, alpha, qc[j], offset_rows=int(rr[i, 1]))
+
+    def cpu_one(j):
+        i = routed[j]
+        arr = cb if rr[i, 0] == 0 else ct
+        s, e = int(rr[i, 1]), int(rr[i, 2])
+        K = min(140, e - s)
+        out = np.zeros(K, np.int32)
+        if R is not None:
+            R.ref_bruteforce(K, orc._p(out), e - s, orc._p(arr[s:e]), alpha, orc._p(qc[j]))
+        else:
+            orc.bf_heap_ref(arr[s:e], qc[j], K)
+        return out
+
+    res = {}
+    with ThreadPoolExecutor(16) as ex:
+        list(ex.map(gpu_one, range(min(64, routed.size))))
+        t = time.perf_counter()
+        got = list(ex.map(gpu_one, range(routed.size)))
+        res["gpu_per_call_16_threads"] = routed.size / (time.perf_counter() - t)
+        t = time.perf_counter()
+        ncpu = min(routed.size, 4000)
+        ref_out = list(ex.map(cpu_one, range(ncpu)))
+        res["cpu_per_call_16_threads"] = ncpu / (time.perf_counter() - t)
+    # the batched form over both layouts (two calls, one per code array)
+    t = time.perf_counter()
+    for L, arr in ((0, cb), (1, ct)):
+        m = rr[routed, 0] == L
+        if m.any():
+            fbf.bruteforce_batch(arr, rr[routed[m], 1].astype(np.uint64), cnt[routed[m]].astype(np.uint64), qc[m], 140)
+    res["gpu_batch"] = routed.size / (time.perf_counter() - t)
+    same = sum(set(got[j].tolist()) == set(ref_out[j].tolist()) for j in range(ncpu))
+    fbf.unregister_codes(cb); fbf.unregister_codes(ct)
+    line = {"dropin": cfg.name, "routed_queries": int(routed.size), "calls_per_s": res,
+            "cpu_kind": "reference" if R is not None else "port", "id_sets_equal": f"{same}/{ncpu}",
+            "mean_slice_rows": float(cnt[routed].mean()), **host_cpu()}
+    print(json.dumps(line), flush=True)
     return 0
 
 
