@@ -64,7 +64,9 @@ typedef enum fbf_mode {
 #define FBF_REPR_BF16 0x4u    /* BF16 MFMA tiles (mode BF16) */
 #define FBF_REPR_FP8 0x8u     /* FP8 e4m3 MFMA tiles + per-row scale (mode FP8) */
 #define FBF_REPR_SQ8SYM 0x10u /* symmetric int8 codes, global absmax alpha (mode SQ8SYM) */
-#define FBF_REPR_ALL 0x1Fu
+#define FBF_REPR_TBUCKETS 0x20u /* the ten 0.1-wide time buckets of the timestamp layout, each with
+                                   its own symmetric int8 codes and alpha (fbf_search_subinterval) */
+#define FBF_REPR_ALL 0x3Fu
 
 typedef struct fbf_dataset fbf_dataset;
 
@@ -146,6 +148,41 @@ fbf_status fbf_search_graph_create(fbf_dataset *ds, const float *queries_dev, co
 fbf_status fbf_search_graph_launch(fbf_search_graph *g, void *stream);
 /* Waits for the device, then frees the graph and its scratch. */
 fbf_status fbf_search_graph_destroy(fbf_search_graph *g);
+
+/* Type-2 sub-interval search: the SMALL-bucket branch of the router's
+ * type-2 path (hybrid_graph.cpp:317-420) and the global refine (:463-494) for
+ * a batch of type-2 queries (metas[q][0] == 2; other types are rejected).
+ * Needs FBF_REPR_TBUCKETS: the time buckets of ReadSortedBaseTimestamp
+ * (utils.h:470-478) over the timestamp layout, bucket t = [lower_bound(0.1 t),
+ * upper_bound(0.1 (t + 1))), each with the symmetric int8 quantiser trained on
+ * its own rows (the timestamp searchers, baseline.cpp:130-141).
+ *
+ * Per query: SplitInterval (hybrid_graph.cpp:91-134) classes each bucket as
+ * FULL (3), MEDIUM (1) or SMALL (0) -- 255 when it does not overlap [l, r] --
+ * with the SMALL threshold of the router (0.5 when the query's selectivity is
+ * below 0.2, else 0.2; opts->small_thr > 0 overrides); for each SMALL bucket
+ * the bucket_k smallest symmetric int8 distances of its slice, the query
+ * encoded with that bucket's alpha (bruteforce_subgraph), ties to the smaller
+ * position; their distances int(d * alpha * alpha) enter the candidate pool,
+ * which keeps the pool_k smallest (pool distance, timestamp-layout position)
+ * (candidate_pool, maxPQIFCS<int>); the pool is reranked with the canonical
+ * FP32 distance and the k smallest (distance, original id) are written
+ * (padded with FBF_PAD_ID / +inf).  FULL and MEDIUM buckets are the reference's
+ * graph searches: the caller may add their candidates as n_ext (position in
+ * the timestamp layout, distance) pairs per query (device arrays [nq][n_ext];
+ * pad position 0xFFFFFFFF), which enter the pool with the same conversion.
+ * bucket_types (host, nq x 10, may be NULL) receives the classes.  queries_dev,
+ * out_ids_dev, out_dists_dev: device; metas: host; asynchronous on stream. */
+typedef struct fbf_subinterval_opts {
+    float small_thr;     /* 0 -> the router's rule above */
+    uint32_t bucket_k;   /* 0 -> 140 (bf_refine_k, hybrid_graph.h:80), at most 896 */
+    uint32_t pool_k;     /* 0 -> 150 (global_graph_refine_k, hybrid_graph.h:84), at most 896 */
+    uint32_t reserved;
+} fbf_subinterval_opts;
+fbf_status fbf_search_subinterval(fbf_dataset *ds, const float *queries_dev, const float *metas, uint64_t nq,
+                                  uint32_t k, const fbf_subinterval_opts *opts, const uint32_t *ext_pos_dev,
+                                  const float *ext_dist_dev, uint32_t n_ext, uint32_t *out_ids_dev,
+                                  float *out_dists_dev, uint8_t *bucket_types, void *stream);
 
 /* Quantiser parameters of the dataset: alpha of the symmetric int8 codes
  * (SQ8SymmetricQuantizer::alpha, sq8_sym_quant.hpp:37-44) and the per-dim

@@ -593,6 +593,135 @@ void orc_router_bf(const int8_t *codes_b, const int8_t *codes_t, const uint32_t 
 }
 
 /* ------------------------------------------------------------------------ */
+/* type-2 sub-interval path (hybrid_graph.cpp:91-134, 317-494)               */
+/* ------------------------------------------------------------------------ */
+
+/* The 0.1-wide time buckets of ReadSortedBaseTimestamp (utils.h:470-478) over
+ * the timestamp-order timestamps ts_t: bucket i = [lower_bound(0.1 i),
+ * upper_bound(0.1 (i + 1))), the float timestamps compared with the double
+ * bounds (so a timestamp equal to a bound lands in both buckets). */
+void orc_time_buckets(const float *ts_t, uint64_t n, uint64_t *start, uint64_t *count) {
+    for (int i = 0; i < 10; ++i) {
+        const double lo = 0.1 * i, hi = 0.1 * (i + 1);
+        uint64_t a = 0, b = n;
+        while (a < b) { uint64_t m = (a + b) / 2; if ((double)ts_t[m] < lo) a = m + 1; else b = m; }
+        const uint64_t s = a;
+        a = 0; b = n;
+        while (a < b) { uint64_t m = (a + b) / 2; if (!(hi < (double)ts_t[m])) a = m + 1; else b = m; }
+        start[i] = s;
+        count[i] = a > s ? a - s : 0;
+    }
+}
+
+/* SplitInterval (hybrid_graph.cpp:91-134) of [l, r] over the buckets, with the
+ * router's threshold rule (hybrid_graph.cpp:324-333): thr = 0.5 when the
+ * query's selectivity (e - s) / nb over the timestamp layout is below 0.2,
+ * else 0.2 (small_thr > 0 replaces the rule).  types[t]: 255 no overlap or no
+ * bucket, 0 SMALL, 1 MEDIUM, 3 FULL (the reference's RFType values);
+ * loc_s / loc_n: the bucket-local start and row count of a partial bucket. */
+void orc_split_interval(float l, float r, const float *ts_t, uint64_t n, uint64_t nb, const uint64_t *bstart,
+                        const uint64_t *bcount, float small_thr, uint8_t *types, uint64_t *loc_s, uint64_t *loc_n) {
+    uint64_t a = 0, b = n;
+    while (a < b) { uint64_t m = (a + b) / 2; if (ts_t[m] < l) a = m + 1; else b = m; }
+    const uint64_t gs = a;
+    a = 0; b = n;
+    while (a < b) { uint64_t m = (a + b) / 2; if (ts_t[m] <= r) a = m + 1; else b = m; }
+    const uint64_t ge = a > gs ? a : gs;
+    const float sel = (float)(ge - gs) / (float)nb;
+    const float thr = small_thr > 0.0f ? small_thr : ((double)sel < 0.2 ? 0.5f : 0.2f);
+    for (int t = 0; t < 10; ++t) {
+        types[t] = 255; loc_s[t] = 0; loc_n[t] = 0;
+        if (bcount[t] == 0) continue;
+        const float ts = (float)t / 10, te = (float)(t + 1) / 10;
+        if (r < ts || l > te) continue;
+        if (l <= ts && r >= te) { types[t] = 3; loc_n[t] = bcount[t]; continue; }
+        const float lb = ts > l ? ts : l, rb = te < r ? te : r;
+        const float *bt = ts_t + bstart[t];
+        const uint64_t bn = bcount[t];
+        a = 0; b = bn;
+        while (a < b) { uint64_t m = (a + b) / 2; if (bt[m] < lb) a = m + 1; else b = m; }
+        const uint64_t s = a;
+        a = 0; b = bn;
+        while (a < b) { uint64_t m = (a + b) / 2; if (bt[m] <= rb) a = m + 1; else b = m; }
+        const uint64_t e = a > s ? a : s;
+        const float bsel = (float)(e - s) / (float)bn;
+        types[t] = bsel < thr ? 0 : 1;
+        loc_s[t] = s;
+        loc_n[t] = e - s;
+    }
+}
+
+/* An int * alpha * alpha pool distance as the reference forms it
+ * (bruteforce.h:287, then the implicit float -> int conversion of
+ * maxPQIFCS<int>::maybe_pop_emplace, hybrid_graph.cpp:415): truncation toward
+ * zero; values at or above 2^31 (undefined there) saturate. */
+static uint32_t pool_dist(int32_t d, float alpha) {
+    const float f = ((float)d * alpha) * alpha;
+    return f >= 2147483648.0f ? 0x7FFFFFFFu : (uint32_t)(int32_t)f;
+}
+
+/* The type-2 sub-interval route for the SMALL buckets (hybrid_graph.cpp:
+ * 317-420) and the global refine (:463-494), with this project's tie rules:
+ * each SMALL bucket's K = min(bucket_k, n_points) smallest (int distance,
+ * position) of its slice of the bucket codes (query encoded with the
+ * bucket's alpha), their distances int(d * alpha * alpha); the pool keeps
+ * the pool_k smallest (pool distance, timestamp-layout position) over the
+ * query's buckets; the refine reranks the pool with the canonical FP32
+ * distance and emits the k smallest (distance, original id), padded.
+ * base: original order; id_t / ts_t: the timestamp layout; bcodes: each
+ * bucket's codes (bcount[t] rows of 112 bytes from row boff[t]); types:
+ * nq x 10 as orc_split_interval; ext_pos / ext_dist: n_ext caller candidates per
+ * query (timestamp-layout positions, pool distances) that join the pool.
+ * Queries must be type 2. */
+void orc_subinterval(const float *base, const uint32_t *id_t, const float *ts_t, uint64_t n, int d,
+                     const int8_t *bcodes, const uint64_t *boff, const uint64_t *bstart, const uint64_t *bcount,
+                     const float *balpha, const float *queries, const float *metas, uint64_t nq, uint64_t nb,
+                     float small_thr, int bucket_k, int pool_k, int k, const uint32_t *ext_pos,
+                     const float *ext_dist, int n_ext, uint32_t *out_ids, float *out_dists, uint8_t *types) {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t qi = 0; qi < (int64_t)nq; ++qi) {
+        uint64_t ls[10], ln[10];
+        orc_split_interval(metas[qi * 4 + 2], metas[qi * 4 + 3], ts_t, n, nb, bstart, bcount, small_thr,
+                           types + qi * 10, ls, ln);
+        const float *q = queries + qi * d;
+        kheap P;
+        P.h = (uint64_t *)malloc(8 * (pool_k > 0 ? pool_k : 1));
+        P.n = 0; P.k = pool_k;
+        int32_t *pos = (int32_t *)malloc(sizeof(int32_t) * (bucket_k > 0 ? bucket_k : 1));
+        int32_t *dd = (int32_t *)malloc(sizeof(int32_t) * (bucket_k > 0 ? bucket_k : 1));
+        for (int t = 0; t < 10; ++t) {
+            if (types[qi * 10 + t] != 0 || ln[t] == 0) continue;
+            int8_t qc[ORC_CODE_DIM];
+            orc_sq8sym_encode(q, 1, d, balpha[t], qc);
+            const int K = (uint64_t)bucket_k < ln[t] ? bucket_k : (int)ln[t];
+            orc_bf_sorted(K, ln[t], bcodes + (boff[t] + ls[t]) * ORC_CODE_DIM, qc, pos, dd);
+            for (int j = 0; j < K; ++j) {
+                const uint32_t p = (uint32_t)(bstart[t] + ls[t] + (uint64_t)pos[j]);
+                kh_push(&P, ((uint64_t)pool_dist(dd[j], balpha[t]) << 32) | p);
+            }
+        }
+        /* the caller's candidates (graph searches of FULL / MEDIUM buckets):
+         * float distance -> int as maybe_pop_emplace(int, int) converts it */
+        for (int j = 0; j < n_ext; ++j) {
+            const uint32_t p = ext_pos[qi * n_ext + j];
+            const float f = ext_dist[qi * n_ext + j];
+            if (p == ORC_PAD_ID || p >= n || !(f >= 0.0f)) continue;
+            const uint32_t pd = f >= 2147483648.0f ? 0x7FFFFFFFu : (uint32_t)(int32_t)f;
+            kh_push(&P, ((uint64_t)pd << 32) | p);
+        }
+        kheap H;
+        H.h = (uint64_t *)malloc(8 * (k > 0 ? k : 1));
+        H.n = 0; H.k = k;
+        for (int j = 0; j < P.n; ++j) {
+            const uint32_t id = id_t[(uint32_t)P.h[j]];
+            kh_push(&H, mk_key(orc_l2_canon(base + (uint64_t)id * d, q, d), id));
+        }
+        kh_emit(&H, k, out_ids + qi * k, out_dists + qi * k);
+        free(H.h); free(P.h); free(pos); free(dd);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* evaluation                                                               */
 /* ------------------------------------------------------------------------ */
 

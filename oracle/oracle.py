@@ -77,6 +77,10 @@ def lib():
         L.orc_exact_ranges_f64.argtypes = [_P, _P, _P, _P, _P, _P, _P, _U64, _I, _P, _P, _U64, _I, _P, _P]
         L.orc_l2_canon_rows.argtypes = [_P, _I, _P, _P, _U64, _I, _P]
         L.orc_bf_heap_ref.argtypes = [_I, _U64, _P, _P, _P, _P]
+        L.orc_time_buckets.argtypes = [_P, _U64, _P, _P]
+        L.orc_split_interval.argtypes = [_F, _F, _P, _U64, _U64, _P, _P, _F, _P, _P, _P]
+        L.orc_subinterval.argtypes = [_P, _P, _P, _U64, _I, _P, _P, _P, _P, _P, _P, _P, _U64, _U64, _F, _I, _I,
+                                      _I, _P, _P, _I, _P, _P, _P]
         L.orc_bf_sorted.argtypes = [_I, _U64, _P, _P, _P, _P]
         L.orc_router_bf.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P, _U64, _I, _F, _P, _P, _U64,
                                     _I, _I, _P, _P]
@@ -123,6 +127,8 @@ def ref():
         R.ref_ground_truth.argtypes = [_P, C.c_uint32, _P, C.c_uint32, _I, _P]
         R.ref_router_bf.argtypes = [_P, _P, _P, _P, _P, _I, _F, _P, _P, _P, C.c_uint32,
                                     C.c_uint32, C.c_uint32, _P]
+        R.ref_subinterval_small.argtypes = [_P, _P, _P, C.c_uint32, _I, _P, _P, C.c_uint32, C.c_uint32,
+                                            C.c_uint32, _P, _P, _P]
         _ref = R
     return _ref
 
@@ -345,6 +351,72 @@ def bf_boundary_tied(lay: Layouts, codes_b, codes_t, qcodes, metas, kq=140) -> n
         _, d = bf_sorted(c[s:e], qcodes[i], kq + 1)
         out[i] = d[kq - 1] == d[kq]
     return out
+
+
+# ---------------------------------------------------------------- type-2 sub-interval path
+class TimeBuckets:
+    """The ten 0.1-wide time buckets of the timestamp layout (utils.h:470-478)
+    with each bucket's symmetric int8 quantiser trained on its own rows
+    (SQ8SymmetricQuantizer via the timestamp searchers' SetData,
+    baseline.cpp:130-141): alpha[t], and codes of bucket t at rows
+    [off[t], off[t] + count[t]) of one array."""
+
+    def __init__(self, base: np.ndarray, lay: Layouts):
+        self.start = np.zeros(10, np.uint64); self.count = np.zeros(10, np.uint64)
+        lib().orc_time_buckets(_p(lay.ts_t), lay.n, _p(self.start), _p(self.count))
+        self.off = np.concatenate([[0], np.cumsum(self.count)[:-1]]).astype(np.uint64)
+        self.alpha = np.zeros(10, np.float32)
+        codes = []
+        for t in range(10):
+            s, c = int(self.start[t]), int(self.count[t])
+            rows = np.ascontiguousarray(base[lay.perm_time[s:s + c]])
+            if c:
+                self.alpha[t] = sq8sym_train(rows)
+                codes.append(sq8sym_encode(rows, float(self.alpha[t])))
+        self.codes = np.ascontiguousarray(np.concatenate(codes) if codes else np.zeros((0, CODE_DIM), np.int8))
+
+    def split(self, lay: Layouts, l, r, nb, small_thr=0.0):
+        types = np.zeros(10, np.uint8); ls = np.zeros(10, np.uint64); ln = np.zeros(10, np.uint64)
+        lib().orc_split_interval(float(l), float(r), _p(lay.ts_t), lay.n, nb, _p(self.start), _p(self.count),
+                                 float(small_thr), _p(types), _p(ls), _p(ln))
+        return types, ls, ln
+
+
+def subinterval(base, lay: Layouts, tb: TimeBuckets, queries, metas, k=100, nb=None, small_thr=0.0,
+                bucket_k=140, pool_k=150, ext=None):
+    """The type-2 SMALL route + candidate pool + global refine restated
+    (orc_subinterval); ext: optional (positions uint32 [nq, n_ext], pool
+    distances float32 [nq, n_ext]).  Returns (ids, dists, types nq x 10)."""
+    base = np.ascontiguousarray(base, np.float32)
+    queries = np.ascontiguousarray(queries, np.float32); metas = np.ascontiguousarray(metas, np.float32)
+    nq = queries.shape[0]
+    ids = np.zeros((nq, k), np.uint32); dists = np.zeros((nq, k), np.float32); types = np.zeros((nq, 10), np.uint8)
+    ep = np.zeros(1, np.uint32); ed = np.zeros(1, np.float32); n_ext = 0
+    if ext is not None:
+        ep = np.ascontiguousarray(ext[0], np.uint32); ed = np.ascontiguousarray(ext[1], np.float32)
+        n_ext = ep.shape[1]
+    lib().orc_subinterval(_p(base), _p(lay.perm_time), _p(lay.ts_t), lay.n, base.shape[1], _p(tb.codes), _p(tb.off),
+                          _p(tb.start), _p(tb.count), _p(tb.alpha), _p(queries), _p(metas), nq,
+                          lay.n if nb is None else nb, float(small_thr), bucket_k, pool_k, k, _p(ep), _p(ed), n_ext,
+                          _p(ids), _p(dists), _p(types))
+    return ids, dists, types
+
+
+def ref_subinterval(base, lay: Layouts, queries, metas, k=100, nb=None):
+    """The reference's own route (oracle/_ref ref_subinterval_small): ids in
+    heap-array order, RFType per bucket (255 none), bucket alphas; None when
+    the reference kernels are not available."""
+    R = ref()
+    if R is None:
+        return None
+    base = np.ascontiguousarray(base, np.float32)
+    vt = np.ascontiguousarray(base[lay.perm_time])
+    queries = np.ascontiguousarray(queries, np.float32); metas = np.ascontiguousarray(metas, np.float32)
+    nq = queries.shape[0]
+    ids = np.zeros((nq, k), np.uint32); types = np.zeros((nq, 10), np.int32); alphas = np.zeros(10, np.float32)
+    R.ref_subinterval_small(_p(vt), _p(lay.ts_t), _p(lay.perm_time), lay.n, base.shape[1], _p(queries), _p(metas), nq,
+                            k, lay.n if nb is None else nb, _p(ids), _p(types), _p(alphas))
+    return ids, types, alphas
 
 
 def recall(knn: np.ndarray, gt: np.ndarray):

@@ -9,11 +9,13 @@
 #include <cstring>
 #include <vector>
 
-#include "bruteforce.h"                     // bruteforce, bruteforce_subgraph, maxPQIFCS
-// distance.h (normal_l2) arrives through utils.h, which has no include guard for it
+// hybrid_graph.cpp brings hybrid_graph.h (HybridGraph::SplitInterval, the
+// timestamp searchers), bruteforce.h (bruteforce, bruteforce_subgraph,
+// maxPQIFCS) and utils.h (Bruteforce, GetKNNRecall, RFType; distance.h's
+// normal_l2 arrives through it) -- compiled from the reference's own files.
+#include "hybrid_graph.cpp"
 #include "glass/quant/sq8_quant.hpp"        // SQ8Quantizer
 #include "glass/quant/sq8_sym_quant.hpp"    // SQ8SymmetricQuantizer
-#include "utils.h"                          // Bruteforce, GetKNNRecall
 
 extern "C" {
 
@@ -110,6 +112,91 @@ void ref_router_bf(const int8_t *codes_b, const int8_t *codes_t, const uint32_t 
             pd.maybe_pop_emplace(tmp[j], glass::L2SqrRef(q, base + (size_t)tmp[j] * d, d));
         for (size_t l = 0; l < k; ++l)
             out_ids[i * k + k - l - 1] = (l < (size_t)pd.size()) ? (uint32_t)pd.data_[l + 1].id : 0xFFFFFFFFu;
+    }
+}
+
+// The type-2 sub-interval route of BatchSearch for the SMALL buckets, driven
+// with the reference's own pieces: the 0.1-wide time buckets of
+// ReadSortedBaseTimestamp (utils.h:470-478), one timestamp searcher per
+// bucket whose SetData trains that bucket's SQ8SymmetricQuantizer
+// (baseline.cpp:130-141, searcher.hpp:94-102), HybridGraph::SplitInterval
+// with the router's threshold rule (hybrid_graph.cpp:324-333), the SMALL
+// branch -- bucket-encoded query, bruteforce_subgraph, candidate_pool
+// maybe_pop_emplace (hybrid_graph.cpp:389-419) -- in bucket order t = 0..9,
+// and the global refine with L2SqrRef into maxPQIFCS<float> (:463-494).
+// FULL and MEDIUM buckets (graph searches there) add nothing.
+// vecs_t / ts_t / ids_t: the rows, timestamps and original ids in timestamp
+// order; nb: the selectivity denominator.  out_ids: nq x k original ids in
+// heap-array order (0xFFFFFFFF where the refine heap holds fewer); out_types:
+// nq x 10 RFType per bucket (255: no overlap or no bucket); alphas: 10.
+void ref_subinterval_small(const float *vecs_t, const float *ts_t, const uint32_t *ids_t, uint32_t n, int d,
+                           const float *queries, const float *metas, uint32_t nq, uint32_t k, uint32_t nb,
+                           uint32_t *out_ids, int32_t *out_types, float *alphas) {
+    HybridGraph h;
+    std::vector<float> tsv(ts_t, ts_t + n);
+    for (int i = 0; i <= 9; i += interval) {
+        auto s = std::lower_bound(tsv.begin(), tsv.end(), 0.1 * i);
+        auto e = std::upper_bound(tsv.begin(), tsv.end(), 0.1 * (i + interval));
+        int num = (int)(e - s);
+        alphas[i] = 0.0f;
+        if (num == 0) continue;
+        h.timestamp_map[i] = {(uint32_t)(s - tsv.begin()), (uint32_t)num};
+    }
+    for (auto &[t, p] : h.timestamp_map) {
+        glass::Graph<int> graph;
+        h.timestamp_searcher[t] = glass::create_searcher(std::move(graph), "L2", 3);
+        h.timestamp_searcher[t]->SetData(vecs_t + (size_t)p.first * d, nullptr, tsv.data() + p.first, p.second, d);
+        alphas[t] = ((glass::Searcher<glass::SQ8SymmetricQuantizer<glass::Metric::L2>> *)h.timestamp_searcher[t].get())
+                        ->quant.alpha;
+    }
+    std::vector<std::unordered_map<int, std::pair<RFType, std::pair<int, int>>>> pool_iv(nq);
+    for (uint32_t i = 0; i < nq; ++i) {
+        const float l = metas[i * 4 + 2], r = metas[i * 4 + 3];
+        auto s = std::lower_bound(tsv.begin(), tsv.end(), l);
+        auto e = std::upper_bound(tsv.begin(), tsv.end(), r);
+        const float sel = (float)(e - s) / (float)nb;
+        if (sel < 0.2) h.SplitInterval(std::make_pair(l, r), pool_iv[i], 0.5);
+        else h.SplitInterval(std::make_pair(l, r), pool_iv[i]);
+        for (int t = 0; t < 10; ++t) {
+            auto it = pool_iv[i].find(t);
+            out_types[i * 10 + t] = it == pool_iv[i].end() ? 255 : (int)it->second.first;
+        }
+    }
+    std::vector<maxPQIFCS<int>> candidate_pool(nq);
+    for (int t = 0; t <= 9; ++t) {
+        auto itm = h.timestamp_map.find(t);
+        if (itm == h.timestamp_map.end()) continue;
+        auto &quant = ((glass::Searcher<glass::SQ8SymmetricQuantizer<glass::Metric::L2>> *)h.timestamp_searcher[t].get())
+                          ->quant;
+        const uint32_t start_id = itm->second.first;
+#pragma omp parallel for schedule(dynamic)
+        for (int64_t i = 0; i < (int64_t)nq; ++i) {
+            auto it = pool_iv[i].find(t);
+            if (it == pool_iv[i].end() || it->second.first != RFType::SMALL) continue;
+            if (candidate_pool[i].size() == 0) candidate_pool[i].resize(h.global_graph_refine_k);
+            const int s = it->second.second.first;
+            const uint32_t n_points = it->second.second.second;
+            int8_t *encode_q = (int8_t *)glass::alloc64B(quant.code_size);
+            quant.encode(queries + (size_t)i * d, (char *)encode_q);
+            char *codes = quant.codes + (size_t)s * quant.code_size;
+            auto K = std::min(h.bf_refine_k, n_points);
+            std::vector<int> tmp(K);
+            std::vector<float> dists(K);
+            bruteforce_subgraph(d, K, tmp.data(), dists.data(), n_points, codes, quant.alpha, 1, encode_q);
+            for (size_t j = 0; j < K; ++j) candidate_pool[i].maybe_pop_emplace(tmp[j] + start_id + s, dists[j]);
+            free(encode_q);
+        }
+    }
+#pragma omp parallel for schedule(dynamic)
+    for (int64_t i = 0; i < (int64_t)nq; ++i) {
+        auto &cd = candidate_pool[i];
+        maxPQIFCS<float> point_dist(k);
+        for (int j = 1; j <= cd.size(); ++j) {
+            const int id = cd.data_[j].id;
+            point_dist.maybe_pop_emplace(id, glass::L2SqrRef(queries + (size_t)i * d, vecs_t + (size_t)id * d, d));
+        }
+        for (uint32_t l = 0; l < k; ++l)
+            out_ids[i * k + k - l - 1] = l < (uint32_t)point_dist.size() ? ids_t[point_dist.data_[l + 1].id] : 0xFFFFFFFFu;
     }
 }
 

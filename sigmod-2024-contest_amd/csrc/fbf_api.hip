@@ -119,6 +119,10 @@ struct Scratch {
         d_tau0, d_flags, d_pre_end;
     // FBF_MODE_EXACT_BF16: per-query certificates and the uncertified subset.
     DevBuf d_cert, d_sub_idx, d_sub_q, d_sub_ids, d_sub_d;
+    // fbf_search_subinterval: slices, candidate pool, pool slot offsets.
+    DevBuf d_ti_sl, d_ti_lists, d_ti_counts, d_ti_off, d_ti_ext;
+    HostPinned h_ti;                 // staging of the slices and pool offsets
+    hipEvent_t ev_ti = nullptr;      // recorded after the copies out of h_ti
     HostPinned h_stage[2], h_nflag;
     hipEvent_t ev_stage[2] = {nullptr, nullptr};  // recorded after the copies out of h_stage[i]
     int stage_next = 0;
@@ -132,8 +136,11 @@ struct Scratch {
     void release() {
         for (DevBuf *b : {&d_items, &d_iq, &d_slot_off, &d_lists, &d_counts, &d_queue, &d_q, &d_out_ids,
                           &d_out_d, &d_gtau, &d_tau0, &d_flags, &d_pre_end, &d_cert, &d_sub_idx, &d_sub_q,
-                          &d_sub_ids, &d_sub_d})
+                          &d_sub_ids, &d_sub_d, &d_ti_sl, &d_ti_lists, &d_ti_counts, &d_ti_off, &d_ti_ext})
             b->release();
+        h_ti.release();
+        if (ev_ti) (void)hipEventDestroy(ev_ti);
+        ev_ti = nullptr;
         for (int i = 0; i < 2; ++i) {
             h_stage[i].release();
             if (ev_stage[i]) (void)hipEventDestroy(ev_stage[i]);
@@ -164,6 +171,13 @@ struct fbf_dataset {
     float max_norm = 0.f;        // largest row norm |x| (rounded up; BF16 builds), for FBF_MODE_EXACT_BF16
     float bf16_rho = 0.f;        // largest |bf16(x) - x| / |x| over the rows (rounded up), same use
     std::vector<float> h_mi, h_dif;
+    // Time buckets of the timestamp layout (FBF_REPR_TBUCKETS): bucket t holds
+    // positions [tb_start[t], tb_start[t] + tb_count[t]) of layout 1, its own
+    // symmetric int8 scale tb_alpha[t], and its codes at rows tb_off[t].. of tb_codes.
+    uint64_t tb_start[10] = {}, tb_count[10] = {}, tb_off[10] = {};
+    float tb_alpha[10] = {};
+    int8_t *tb_codes = nullptr;
+    uint64_t tb_rows = 0;
     Scratch scr;
     std::mutex mu;
     std::mutex mu_cert;          // one FBF_MODE_EXACT_BF16 search at a time (its subset buffers)
@@ -374,6 +388,43 @@ static fbf_status dataset_build(fbf_dataset *ds, const float *labels, const floa
                                       D.sym_mfma, nullptr, st));
         }
     }
+    if (repr & FBF_REPR_TBUCKETS) {
+        // The timestamp sub-indexes' buckets (ReadSortedBaseTimestamp,
+        // utils.h:470-478: bucket i = [lower_bound(0.1 i), upper_bound(0.1 (i +
+        // 1))) of the timestamp layout, float timestamps against double bounds)
+        // and their quantisers, each trained on its own rows (the timestamp
+        // searchers' SetData, baseline.cpp:130-141).
+        const float *t = ds->ts_t.data();
+        uint64_t total = 0;
+        for (int b = 0; b < 10; ++b) {
+            const double lo = 0.1 * b, hi = 0.1 * (b + 1);
+            const uint64_t s0 = std::lower_bound(t, t + n, lo, [](float a, double v) { return (double)a < v; }) - t;
+            const uint64_t e0 = std::upper_bound(t, t + n, hi, [](double v, float a) { return v < (double)a; }) - t;
+            ds->tb_start[b] = s0;
+            ds->tb_count[b] = e0 > s0 ? e0 - s0 : 0;
+            ds->tb_off[b] = total;
+            total += ds->tb_count[b];
+        }
+        ds->tb_rows = total;
+        HIP_TRY(hipMalloc(&ds->tb_codes, std::max<uint64_t>(1, total) * 112));
+        HIP_TRY(absbits.reserve(sizeof(uint32_t)));
+        for (int b = 0; b < 10; ++b) {
+            const uint64_t cnt = ds->tb_count[b];
+            if (cnt == 0) continue;
+            const uint32_t *perm = ds->lay[1].ids + ds->tb_start[b];
+            HIP_TRY(hipMemsetAsync(absbits.p, 0, sizeof(uint32_t), st));
+            HIP_TRY(launch_absmax_rows(ds->base_rows, perm, cnt, absbits.as<uint32_t>(), st));
+            uint32_t bits = 0;
+            HIP_TRY(hipMemcpyAsync(&bits, absbits.p, 4, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            std::memcpy(&ds->tb_alpha[b], &bits, 4);
+            int8_t *out = ds->tb_codes + ds->tb_off[b] * 112;
+            // A bucket of all-zero rows has scale 0 (the reference divides by
+            // it); its codes are defined as zero.
+            if (ds->tb_alpha[b] > 0.f) HIP_TRY(launch_sym_encode(ds->base_rows, perm, cnt, cnt, ds->tb_alpha[b], out, st));
+            else HIP_TRY(hipMemsetAsync(out, 0, cnt * 112, st));
+        }
+    }
     record_end(ds, st, ev);
     HIP_TRY(hipStreamSynchronize(st));
     norms.release();
@@ -436,6 +487,7 @@ extern "C" fbf_status fbf_dataset_destroy(fbf_dataset *ds) {
     for (auto &L : ds->lay) L.free_all();
     for (float *p : {ds->sq8_mi, ds->sq8_dif, ds->sq8_w})
         if (p) (void)hipFree(p);
+    if (ds->tb_codes) (void)hipFree(ds->tb_codes);
     ds->scr.release();
     for (auto &ev : ds->events) {
         if (ev.a) hipEventDestroy(ev.a);
@@ -1429,7 +1481,9 @@ extern "C" fbf_status fbf_last_search_fallbacks(const fbf_dataset *ds, uint64_t 
 
 extern "C" const char *fbf_last_error(void) { return g_err.c_str(); }
 
-extern "C" const char *fbf_version(void) { return "fbf 0.2 gfx950: exact-f32 bf16-mfma fp8-mfma sq8-mfma sq8sym"; }
+extern "C" const char *fbf_version(void) {
+    return "fbf 0.3 gfx950: exact-f32 bf16-mfma fp8-mfma sq8-mfma sq8sym subinterval noalloc";
+}
 
 // ---------------------------------------------------------------- reference-shaped entry points
 //
@@ -1829,5 +1883,189 @@ extern "C" fbf_status fbf_bruteforce_batch_noalloc(const char *codes_base, const
     HIP_TRY(hipStreamSynchronize(st));
     sym_unpack(reinterpret_cast<const uint32_t *>(out_pos), reinterpret_cast<const uint32_t *>(out_dist), starts, nq, k,
                out_pos, out_dist);
+    return FBF_OK;
+}
+
+// ---------------------------------------------------------------- type-2 sub-interval path
+//
+// fbf_search_subinterval: the SMALL-bucket branch of the router's type-2
+// search (hybrid_graph.cpp:317-420) and its global refine (:463-494) for a
+// batch.  Per query, SplitInterval over the time buckets (host, O(10 log n));
+// every SMALL (query, bucket) slice becomes one slice of the buckets' code
+// array, scanned by the symmetric int8 kernel with the query encoded by that
+// bucket's scale and cut to its bucket_k best; the slices' results (and any
+// caller candidates) form each query's candidate pool of pool_k, and the
+// merge reranks the pool with the canonical FP32 distance.
+
+// SplitInterval (hybrid_graph.cpp:91-134) of [l, r]: types[t] 255 (bucket
+// absent or not overlapped), 0 SMALL, 1 MEDIUM, 3 FULL; ls / ln the local
+// start and count of a partial bucket.  thr: the SMALL threshold.
+static void split_interval(const fbf_dataset *ds, float l, float r, float thr, uint8_t *types, uint64_t *ls,
+                           uint64_t *ln) {
+    const float *ts = ds->ts_t.data();
+    for (int t = 0; t < 10; ++t) {
+        types[t] = 255;
+        ls[t] = ln[t] = 0;
+        const uint64_t bn = ds->tb_count[t];
+        if (bn == 0) continue;
+        const float b_lo = (float)t / 10, b_hi = (float)(t + 1) / 10;
+        if (r < b_lo || l > b_hi) continue;
+        if (l <= b_lo && r >= b_hi) {
+            types[t] = 3;
+            ln[t] = bn;
+            continue;
+        }
+        const float lb = std::max(b_lo, l), rb = std::min(b_hi, r);
+        const float *bt = ts + ds->tb_start[t];
+        const uint64_t s = std::lower_bound(bt, bt + bn, lb) - bt;
+        const uint64_t e = std::max<uint64_t>(s, std::upper_bound(bt, bt + bn, rb) - bt);
+        types[t] = (float)(e - s) / (float)bn < thr ? 0 : 1;
+        ls[t] = s;
+        ln[t] = e - s;
+    }
+}
+
+extern "C" fbf_status fbf_search_subinterval(fbf_dataset *ds, const float *queries_dev, const float *metas,
+                                             uint64_t nq, uint32_t k, const fbf_subinterval_opts *opts,
+                                             const uint32_t *ext_pos_dev, const float *ext_dist_dev, uint32_t n_ext,
+                                             uint32_t *out_ids_dev, float *out_dists_dev, uint8_t *bucket_types,
+                                             void *stream) {
+    if (!ds || !metas || (nq && (!queries_dev || !out_ids_dev || !out_dists_dev)))
+        return fail(FBF_ERR_INVALID_ARG, "null argument");
+    if (!(ds->repr & FBF_REPR_TBUCKETS)) return fail(FBF_ERR_NO_REPR, "dataset was created without FBF_REPR_TBUCKETS");
+    if (k == 0 || k > 512) return fail(FBF_ERR_INVALID_ARG, "k must be in [1, 512]");
+    if (n_ext && (!ext_pos_dev || !ext_dist_dev)) return fail(FBF_ERR_INVALID_ARG, "null external candidates");
+    if (n_ext > 896) return fail(FBF_ERR_INVALID_ARG, "n_ext must be <= 896");
+    fbf_subinterval_opts o{};
+    if (opts) o = *opts;
+    const uint32_t kb = o.bucket_k ? o.bucket_k : 140u;   // bf_refine_k, hybrid_graph.h:80
+    const uint32_t kp = o.pool_k ? o.pool_k : 150u;       // global_graph_refine_k, hybrid_graph.h:84
+    if (kb > 896 || kp > 896) return fail(FBF_ERR_INVALID_ARG, "bucket_k and pool_k must be <= 896");
+    if (!(o.small_thr >= 0.f)) return fail(FBF_ERR_INVALID_ARG, "small_thr must be >= 0");
+    if (nq == 0) return FBF_OK;
+    // Split every query on the host.
+    std::vector<SliceInfo> sl;
+    std::vector<uint64_t> starts, counts;
+    std::vector<uint64_t> pool_off(nq + 1, 0);
+    std::vector<uint32_t> ext_slot(nq, 0);
+    std::vector<uint8_t> types(nq * 10);
+    uint64_t slot = 0;
+    for (uint64_t q = 0; q < nq; ++q) {
+        const float *m = metas + 4 * q;
+        if (!(m[0] == 2.0f)) return fail(FBF_ERR_INVALID_ARG, "query " + std::to_string(q) + ": not a type-2 filter");
+        const float l = m[2], r = m[3];
+        // The query's selectivity over the timestamp layout picks the SMALL
+        // threshold (hybrid_graph.cpp:324-333): 0.5 below 0.2, else 0.2.
+        const float *ts = ds->ts_t.data();
+        const uint64_t gs = std::lower_bound(ts, ts + ds->n, l) - ts;
+        const uint64_t ge = std::max<uint64_t>(gs, std::upper_bound(ts, ts + ds->n, r) - ts);
+        const float sel = (float)(ge - gs) / (float)ds->n;
+        const float thr = o.small_thr > 0.f ? o.small_thr : ((double)sel < 0.2 ? 0.5f : 0.2f);
+        uint64_t ls[10], ln[10];
+        split_interval(ds, l, r, thr, &types[q * 10], ls, ln);
+        pool_off[q] = slot;
+        for (int t = 0; t < 10; ++t) {
+            if (types[q * 10 + t] != 0 || ln[t] == 0) continue;
+            SliceInfo S{};
+            S.q = (uint32_t)q;
+            S.alpha = ds->tb_alpha[t];
+            S.row_to_pos = (int64_t)ds->tb_start[t] - (int64_t)ds->tb_off[t];
+            S.pool_slot = (uint32_t)slot++;
+            sl.push_back(S);
+            starts.push_back(ds->tb_off[t] + ls[t]);
+            counts.push_back(ln[t]);
+        }
+        if (n_ext) ext_slot[q] = (uint32_t)slot++;
+    }
+    pool_off[nq] = slot;
+    if (bucket_types) std::memcpy(bucket_types, types.data(), types.size());
+    const uint64_t nsl = sl.size();
+    const uint32_t stride = std::max(kb, n_ext);
+
+    std::lock_guard<std::mutex> lock(ds->mu);
+    HIP_TRY(hipSetDevice(ds->device));
+    hipStream_t st = (hipStream_t)stream;
+    Scratch &S = ds->scr;
+    struct DoneRecord {
+        hipEvent_t ev = nullptr;
+        hipStream_t st = nullptr;
+        ~DoneRecord() {
+            if (ev) (void)hipEventRecord(ev, st);
+        }
+    } done_rec;
+    if (!S.ev_done) HIP_TRY(hipEventCreateWithFlags(&S.ev_done, hipEventDisableTiming));
+    else if (S.used) HIP_TRY(hipStreamWaitEvent(st, S.ev_done, 0));
+    S.used = true;
+    done_rec.ev = S.ev_done;
+    done_rec.st = st;
+
+    // The slices' scan: plan, query codes, scan and per-slice merge to kb.
+    Plan P;
+    fbf_status s = sym_plan(starts.data(), counts.data(), nsl, ds->tb_rows, P);
+    if (s != FBF_OK) return s;
+    const int B = list_capacity((int)kb);
+    s = upload_plan(S, P, nsl, B, st);
+    if (s != FBF_OK) return s;
+    HIP_TRY(S.d_q.reserve(std::max<uint64_t>(1, nsl) * 112));
+    HIP_TRY(S.d_out_ids.reserve(std::max<uint64_t>(1, nsl) * kb * 4));
+    HIP_TRY(S.d_out_d.reserve(std::max<uint64_t>(1, nsl) * kb * 4));
+    HIP_TRY(S.d_ti_sl.reserve(std::max<uint64_t>(1, nsl) * sizeof(SliceInfo)));
+    HIP_TRY(S.d_ti_lists.reserve(std::max<uint64_t>(1, slot) * stride * sizeof(uint64_t)));
+    HIP_TRY(S.d_ti_counts.reserve(std::max<uint64_t>(1, slot) * sizeof(uint32_t)));
+    HIP_TRY(S.d_ti_off.reserve((nq + 1) * sizeof(uint64_t)));
+    HIP_TRY(S.d_ti_ext.reserve(nq * sizeof(uint32_t)));
+    // Slices, pool offsets and ext slots through one pinned staging buffer,
+    // refilled only after the copies of the previous call are done.
+    const size_t b_sl = nsl * sizeof(SliceInfo), b_off = (nq + 1) * sizeof(uint64_t), b_ext = nq * sizeof(uint32_t);
+    if (S.ev_ti) HIP_TRY(hipEventSynchronize(S.ev_ti));
+    else HIP_TRY(hipEventCreateWithFlags(&S.ev_ti, hipEventDisableTiming));
+    HIP_TRY(S.h_ti.reserve(b_sl + b_off + b_ext + 64));
+    char *h = (char *)S.h_ti.p;
+    if (b_sl) std::memcpy(h, sl.data(), b_sl);
+    std::memcpy(h + b_sl, pool_off.data(), b_off);
+    std::memcpy(h + b_sl + b_off, ext_slot.data(), b_ext);
+    if (b_sl) HIP_TRY(hipMemcpyAsync(S.d_ti_sl.p, h, b_sl, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(S.d_ti_off.p, h + b_sl, b_off, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(S.d_ti_ext.p, h + b_sl + b_off, b_ext, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipEventRecord(S.ev_ti, st));
+    if (nsl) {
+        HIP_TRY(launch_sym_encode_slices(queries_dev, S.d_ti_sl.as<SliceInfo>(), nsl, S.d_q.as<int8_t>(), st));
+        SymBufs b;
+        b.items = S.d_items.as<WorkItem>();
+        b.iq = S.d_iq.as<ItemQuery>();
+        b.slot_off = S.d_slot_off.as<uint64_t>();
+        b.lists = S.d_lists.as<uint64_t>();
+        b.counts = S.d_counts.as<uint32_t>();
+        b.qcodes = S.d_q.as<int8_t>();
+        b.out_ids = S.d_out_ids.as<uint32_t>();
+        b.out_d = S.d_out_d.as<float>();
+        s = sym_launch(ds->tb_codes, ds->tb_rows, P, nsl, kb, B, b, st);
+        if (s != FBF_OK) return s;
+        HIP_TRY(launch_pool_keys(S.d_out_ids.as<uint32_t>(), S.d_out_d.as<uint32_t>(), S.d_ti_sl.as<SliceInfo>(), nsl,
+                                 kb, stride, S.d_ti_lists.as<uint64_t>(), S.d_ti_counts.as<uint32_t>(), st));
+    }
+    if (n_ext)
+        HIP_TRY(launch_ext_keys(ext_pos_dev, ext_dist_dev, S.d_ti_ext.as<uint32_t>(), nq, n_ext, ds->n, stride,
+                                S.d_ti_lists.as<uint64_t>(), S.d_ti_counts.as<uint32_t>(), st));
+    // The pool (pool_k smallest keys over the query's slots) and the global
+    // refine (canonical FP32, k smallest (distance, original id)).
+    MergeArgs m{};
+    m.slot_off = S.d_ti_off.as<uint64_t>();
+    m.lists = S.d_ti_lists.as<uint64_t>();
+    m.counts = S.d_ti_counts.as<uint32_t>();
+    m.list_cap = stride;
+    m.nq = nq;
+    m.k = k;
+    m.kprime = kp;
+    m.queries = queries_dev;
+    m.base_rows = ds->base_rows;
+    m.out_ids = out_ids_dev;
+    m.out_dists = out_dists_dev;
+    m.rerank = 1;
+    m.key_int = 0;
+    m.key_pos = 1;
+    m.ids_lay[0] = ds->lay[0].ids;
+    m.ids_lay[1] = ds->lay[1].ids;
+    HIP_TRY(launch_merge(m, st));
     return FBF_OK;
 }

@@ -181,6 +181,24 @@ struct MergeArgs {
     uint32_t *nflag;
 };
 
+// One SMALL (query, time bucket) slice of the type-2 sub-interval path: its
+// query, its bucket's alpha, the offset from a bucket-code row to its
+// timestamp-layout position, and its slot in the candidate pool.
+struct SliceInfo {
+    uint32_t q;
+    float alpha;
+    int64_t row_to_pos;
+    uint32_t pool_slot;
+    uint32_t pad;
+};
+
+hipError_t launch_absmax_rows(const float *rows, const uint32_t *perm, uint64_t n, uint32_t *out_bits, hipStream_t st);
+hipError_t launch_sym_encode_slices(const float *queries, const SliceInfo *sl, uint64_t n, int8_t *out, hipStream_t st);
+hipError_t launch_pool_keys(const uint32_t *ids, const uint32_t *dbits, const SliceInfo *sl, uint64_t n, uint32_t kb,
+                            uint32_t stride, uint64_t *lists, uint32_t *counts, hipStream_t st);
+hipError_t launch_ext_keys(const uint32_t *pos, const float *dist, const uint32_t *ext_slot, uint64_t nq, uint32_t n_ext,
+                           uint64_t n_pos, uint32_t stride, uint64_t *lists, uint32_t *counts, hipStream_t st);
+
 hipError_t launch_scan_f32(const ScanArgs &a, int B, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st);
 hipError_t launch_merge(const MergeArgs &a, hipStream_t st);

@@ -209,3 +209,129 @@ hipError_t launch_sym_encode(const float *rows, const uint32_t *perm, uint64_t n
 }
 
 }  // namespace fbf
+
+namespace fbf {
+
+// ---------------------------------------------------------------- type-2 sub-interval path
+
+// Absmax over the rows perm[0..n) of rows (one time bucket's quantiser,
+// SQ8SymmetricQuantizer::train over that bucket's rows, sq8_sym_quant.hpp:32-44).
+__global__ void absmax_rows_kernel(const float *__restrict__ rows, const uint32_t *__restrict__ perm, uint64_t n,
+                                   uint32_t *__restrict__ out_bits) {
+    uint32_t m = 0;
+    const uint64_t total = n * kDim;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (uint64_t)gridDim.x * blockDim.x)
+        m = max(m, __builtin_bit_cast(uint32_t, fabsf(rows[(uint64_t)perm[i / kDim] * kDim + i % kDim])));
+    for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(out_bits, m);
+}
+
+hipError_t launch_absmax_rows(const float *rows, const uint32_t *perm, uint64_t n, uint32_t *out_bits, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const uint64_t blocks = std::min<uint64_t>(1024, (n * kDim + 255) / 256);
+    hipLaunchKernelGGL(absmax_rows_kernel, dim3((uint32_t)blocks), dim3(256), 0, st, rows, perm, n, out_bits);
+    return hipGetLastError();
+}
+
+// Query codes of each slice: query sl[i].q encoded with its bucket's alpha
+// (the bucket quantiser's encode, hybrid_graph.cpp:402-405), 112 bytes.
+__global__ void sym_encode_slices_kernel(const float *__restrict__ queries, const SliceInfo *__restrict__ sl,
+                                         uint64_t n, int4 *__restrict__ out) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * kCodeInt4) return;
+    const uint64_t i = t / kCodeInt4;
+    const int c = (int)(t % kCodeInt4);
+    const SliceInfo S = sl[i];
+    const float *x = queries + (uint64_t)S.q * kDim;
+    uint32_t w[4] = {0, 0, 0, 0};
+    for (int e = 0; e < 16; ++e) {
+        const int k = 16 * c + e;
+        int8_t code = 0;
+        if (k < kDim) {
+            float v = x[k] / S.alpha;
+            if (v > 1.0f) v = 1.0f;
+            if (v < -1.0f) v = -1.0f;
+            code = (int8_t)roundf(v * 127.0f);
+        }
+        w[e / 4] |= (uint32_t)(uint8_t)code << (8 * (e % 4));
+    }
+    out[t] = make_int4((int)w[0], (int)w[1], (int)w[2], (int)w[3]);
+}
+
+hipError_t launch_sym_encode_slices(const float *queries, const SliceInfo *sl, uint64_t n, int8_t *out, hipStream_t st) {
+    const uint64_t total = n * kCodeInt4;
+    if (total == 0) return hipSuccess;
+    hipLaunchKernelGGL(sym_encode_slices_kernel, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, st, queries, sl,
+                       n, reinterpret_cast<int4 *>(out));
+    return hipGetLastError();
+}
+
+// A pool distance as the reference forms it: the int distance times alpha
+// times alpha in float (bruteforce.h:287), converted to int by the pool's
+// maybe_pop_emplace(int, int) (truncation toward zero, hybrid_graph.cpp:415);
+// values at or above 2^31 saturate.
+__device__ inline uint32_t pool_dist_of(float f) {
+    return f >= 2147483648.0f ? 0x7FFFFFFFu : (uint32_t)(int32_t)f;
+}
+
+// Pool keys of the SMALL slices: slice i's kb merged (int distance, code row)
+// results become (pool distance, 1 << 31 | timestamp-layout position) keys in
+// its pool slot; pads stay kKeyMax (never offered).  Slot counts are kb.
+__global__ void pool_keys_kernel(const uint32_t *__restrict__ ids, const uint32_t *__restrict__ dbits,
+                                 const SliceInfo *__restrict__ sl, uint64_t n, uint32_t kb, uint32_t stride,
+                                 uint64_t *__restrict__ lists, uint32_t *__restrict__ counts) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * kb) return;
+    const uint64_t i = t / kb;
+    const uint32_t j = (uint32_t)(t % kb);
+    const SliceInfo S = sl[i];
+    const uint32_t id = ids[t];
+    uint64_t key = kKeyMax;
+    if (id != kPadId) {
+        const float f = ((float)(int32_t)dbits[t] * S.alpha) * S.alpha;
+        const uint32_t pos = (uint32_t)((int64_t)id + S.row_to_pos);
+        key = ((uint64_t)pool_dist_of(f) << 32) | 0x80000000u | pos;
+    }
+    lists[(uint64_t)S.pool_slot * stride + j] = key;
+    if (j == 0) counts[S.pool_slot] = kb;
+}
+
+hipError_t launch_pool_keys(const uint32_t *ids, const uint32_t *dbits, const SliceInfo *sl, uint64_t n, uint32_t kb,
+                            uint32_t stride, uint64_t *lists, uint32_t *counts, hipStream_t st) {
+    const uint64_t total = n * kb;
+    if (total == 0) return hipSuccess;
+    hipLaunchKernelGGL(pool_keys_kernel, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, st, ids, dbits, sl, n,
+                       kb, stride, lists, counts);
+    return hipGetLastError();
+}
+
+// Pool keys of the caller's candidates (the graph searches of FULL / MEDIUM
+// buckets, hybrid_graph.cpp:421-458): query q's n_ext (position, distance)
+// pairs into its ext slot, distances converted as the pool converts them;
+// pad ids (0xFFFFFFFF) and positions at or beyond n_pos are not offered.
+__global__ void ext_keys_kernel(const uint32_t *__restrict__ pos, const float *__restrict__ dist,
+                                const uint32_t *__restrict__ ext_slot, uint64_t nq, uint32_t n_ext, uint64_t n_pos,
+                                uint32_t stride, uint64_t *__restrict__ lists, uint32_t *__restrict__ counts) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nq * n_ext) return;
+    const uint64_t q = t / n_ext;
+    const uint32_t j = (uint32_t)(t % n_ext);
+    const uint32_t p = pos[t];
+    const float f = dist[t];
+    uint64_t key = kKeyMax;
+    if (p != kPadId && (uint64_t)p < n_pos && f >= 0.0f) key = ((uint64_t)pool_dist_of(f) << 32) | 0x80000000u | p;
+    const uint32_t slot = ext_slot[q];
+    lists[(uint64_t)slot * stride + j] = key;
+    if (j == 0) counts[slot] = n_ext;
+}
+
+hipError_t launch_ext_keys(const uint32_t *pos, const float *dist, const uint32_t *ext_slot, uint64_t nq, uint32_t n_ext,
+                           uint64_t n_pos, uint32_t stride, uint64_t *lists, uint32_t *counts, hipStream_t st) {
+    const uint64_t total = nq * n_ext;
+    if (total == 0) return hipSuccess;
+    hipLaunchKernelGGL(ext_keys_kernel, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, st, pos, dist, ext_slot,
+                       nq, n_ext, n_pos, stride, lists, counts);
+    return hipGetLastError();
+}
+
+}  // namespace fbf

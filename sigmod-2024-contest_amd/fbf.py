@@ -28,8 +28,8 @@ CODE_BYTES = 112
 MODE_EXACT_F32, MODE_SQ8, MODE_BF16, MODE_FP8, MODE_SQ8SYM, MODE_EXACT_BF16 = range(6)
 MODES = {"exact": MODE_EXACT_F32, "sq8": MODE_SQ8, "bf16": MODE_BF16, "fp8": MODE_FP8,
          "sq8sym": MODE_SQ8SYM, "exact_bf16": MODE_EXACT_BF16}
-REPR_F32, REPR_SQ8, REPR_BF16, REPR_FP8, REPR_SQ8SYM = 0x1, 0x2, 0x4, 0x8, 0x10
-REPR_ALL = 0x1F
+REPR_F32, REPR_SQ8, REPR_BF16, REPR_FP8, REPR_SQ8SYM, REPR_TBUCKETS = 0x1, 0x2, 0x4, 0x8, 0x10, 0x20
+REPR_ALL = 0x3F
 MODE_REPR = {MODE_EXACT_F32: REPR_F32, MODE_SQ8: REPR_SQ8, MODE_BF16: REPR_BF16,
              MODE_FP8: REPR_FP8, MODE_SQ8SYM: REPR_SQ8SYM, MODE_EXACT_BF16: REPR_BF16 | REPR_F32}
 
@@ -41,12 +41,17 @@ EXPORTS = [
     "fbf_profile_enable", "fbf_profile_collect", "fbf_last_search_stats", "fbf_last_search_rescans",
     "fbf_last_error", "fbf_version", "fbf_search_graph_create", "fbf_search_graph_launch",
     "fbf_search_graph_destroy", "fbf_last_search_fallbacks", "fbf_dataset_layout_rows",
-    "fbf_last_search_pass_pairs", "fbf_bruteforce_batch_noalloc",
+    "fbf_last_search_pass_pairs", "fbf_bruteforce_batch_noalloc", "fbf_search_subinterval",
 ]
 
 
 class FbfError(RuntimeError):
     pass
+
+
+class SubintervalOpts(C.Structure):
+    _fields_ = [("small_thr", C.c_float), ("bucket_k", C.c_uint32), ("pool_k", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 class SearchOpts(C.Structure):
@@ -84,6 +89,8 @@ def lib() -> C.CDLL:
                                           C.c_size_t, _P]
     L.fbf_bruteforce_batch.argtypes = [_P, _P, _P, _P, C.c_uint64, C.c_uint32, _P, _P]
     L.fbf_bruteforce_batch_noalloc.argtypes = [_P, _P, _P, _P, C.c_uint64, C.c_uint32, _P, _P, _P]
+    L.fbf_search_subinterval.argtypes = [_P, _P, _P, C.c_uint64, C.c_uint32, C.POINTER(SubintervalOpts), _P, _P,
+                                         C.c_uint32, _P, _P, _P, _P]
     L.fbf_profile_enable.argtypes = [_P, i]
     L.fbf_profile_collect.argtypes = [_P, C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]
     L.fbf_last_search_stats.argtypes = [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
@@ -217,6 +224,35 @@ class Dataset:
         _check(lib().fbf_search_host(self.h, _np_ptr(queries), _np_ptr(metas), nq, k, C.byref(o),
                                      _np_ptr(ids), _np_ptr(dists)), "fbf_search_host")
         return ids, dists
+
+    def search_subinterval(self, queries, metas: np.ndarray, k: int = 100, small_thr: float = 0.0,
+                           bucket_k: int = 0, pool_k: int = 0, ext=None, stream=None):
+        """Type-2 sub-interval search (fbf_search_subinterval): queries torch
+        CUDA float32 [nq, d], metas numpy [nq, 4] (type 2).  ext: optional
+        (positions uint32/int32 CUDA [nq, n_ext], distances float32 CUDA
+        [nq, n_ext]) candidates of the graph-searched buckets.  Returns torch
+        (ids, dists) and numpy bucket classes [nq, 10] (255 none, 0 SMALL,
+        1 MEDIUM, 3 FULL)."""
+        import torch
+        metas = np.ascontiguousarray(metas, np.float32)
+        nq = metas.shape[0]
+        assert queries.shape == (nq, self.d) and queries.dtype == torch.float32
+        ids = torch.empty((nq, k), dtype=torch.int32, device=queries.device)
+        dists = torch.empty((nq, k), dtype=torch.float32, device=queries.device)
+        types = np.zeros((nq, 10), np.uint8)
+        o = SubintervalOpts()
+        o.small_thr, o.bucket_k, o.pool_k = small_thr, bucket_k, pool_k
+        n_ext, ep, ed = 0, None, None
+        if ext is not None:
+            ep, ed = ext
+            assert ep.shape == ed.shape and ep.shape[0] == nq
+            n_ext = ep.shape[1]
+            ep, ed = _P(_dev_ptr(ep)), _P(_dev_ptr(ed))
+        st = (stream or torch.cuda.current_stream(queries.device)).cuda_stream
+        _check(lib().fbf_search_subinterval(self.h, _P(_dev_ptr(queries)), _np_ptr(metas), nq, k, C.byref(o), ep, ed,
+                                            n_ext, _P(_dev_ptr(ids)), _P(_dev_ptr(dists)), _np_ptr(types), _P(st)),
+               "fbf_search_subinterval")
+        return ids, dists, types
 
     def quant_params(self):
         """(alpha, mi, dif): the symmetric int8 scale (None without

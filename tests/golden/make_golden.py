@@ -18,6 +18,12 @@ were made).
   203-228: bruteforce() top-140 of the symmetric int8 codes, id map, L2SqrRef
   rerank to 100 in maxPQIFCS) run by the reference kernels over every dummy
   query's filtered range of the sorted layouts; ids in heap-array order.
+* ref_subinterval.npz: the reference's type-2 sub-interval route for the
+  SMALL buckets (its SplitInterval, per-bucket quantisers, bruteforce_subgraph,
+  candidate pool and L2SqrRef refine, oracle/ref_shim.cpp
+  ref_subinterval_small) on the config-0 base at 20,000 rows (datagen seed)
+  for 400 type-2 queries; ids in heap-array order, RFType per bucket, the
+  bucket alphas.
 """
 from __future__ import annotations
 
@@ -94,6 +100,23 @@ def main() -> None:
                     orc._p(qcodes_all), orc._p(np.ascontiguousarray(ranges)), nq, 140, 100, orc._p(rr_ids))
     np.savez_compressed(os.path.join(HERE, "ref_router_bf.npz"), ranges=ranges, ids=rr_ids, kq=140, k=100,
                         alpha=np.float32(alpha))
+
+    # The type-2 sub-interval route on a uniform-timestamp base.
+    sv, sl, st = dg.make_base(dg.CONFIGS[0], n=20000)
+    Ls = orc.Layouts(sl.astype(np.int32), st)
+    rng = np.random.default_rng(41)
+    nq2 = 400
+    q2, _ = dg.make_queries(dg.CONFIGS[0], nq=nq2)
+    l2 = rng.random(nq2).astype(np.float32)
+    w2 = np.exp(rng.uniform(np.log(0.005), np.log(0.5), nq2)).astype(np.float32)
+    m2 = np.full((nq2, 4), -1.0, np.float32)
+    m2[:, 0] = 2
+    m2[:, 2] = l2
+    m2[:, 3] = np.minimum(l2 + w2, np.float32(1.0))
+    m2[:4, 2:] = [[0.1, 0.2], [0.0, 1.0], [0.25, 0.26], [0.5, 0.5]]  # whole bucket, everything, tiny, one instant
+    rid, rtypes, ralpha = orc.ref_subinterval(sv, Ls, q2, m2, 100)
+    np.savez_compressed(os.path.join(HERE, "ref_subinterval.npz"), n=20000, queries=q2, metas=m2, ids=rid,
+                        types=rtypes, alphas=ralpha)
 
     a0 = [127, 127, 127, -4, 5, 6, 7, 8, 9, -127, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, -24,
           25, 26, 27, 28, 29, 30, 31, 32]

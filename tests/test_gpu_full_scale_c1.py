@@ -149,3 +149,27 @@ def test_c1_max_batch_split_equals_whole(c1):
     a = _search(c1, mode="bf16")
     b = _search(c1, mode="bf16", max_batch=30000)
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_c1_subinterval_graph_routed_type2(orc, c1):
+    """The type-2 sub-interval path on the whole batch's type-2 queries that
+    the router sends to the graph side (selectivity >= 0.045): bit-exact
+    against the oracle on a sample of them."""
+    import torch
+    m = c1["metas"]
+    sel = c1["counts"] / float(c1["vecs"].shape[0])
+    idx = np.nonzero((m[:, 0] == 2) & (sel >= 0.045))[0]
+    _say(f"case: N=10M sub-interval search over {idx.size} graph-routed type-2 queries")
+    qd = c1["qd"][torch.from_numpy(idx).cuda()].contiguous()
+    ids, d, types = c1["ds"].search_subinterval(qd, m[idx])
+    torch.cuda.synchronize()
+    ids = ids.cpu().numpy().view(np.uint32)
+    d = d.cpu().numpy()
+    tb = orc.TimeBuckets(c1["vecs"], c1["L"])
+    s = np.random.default_rng(61).choice(idx.size, 200, replace=False)
+    oid, od, ot = orc.subinterval(c1["vecs"], c1["L"], tb, c1["q"][idx[s]], m[idx[s]], K)
+    _say(f"  {int((types == 0).sum())} SMALL slices over the batch; {int((ot == 0).any(1).sum())} of the 200 "
+         f"checked queries have one")
+    assert np.array_equal(types[s], ot)
+    assert np.array_equal(ids[s], oid)
+    assert np.array_equal(d[s], od)
