@@ -94,6 +94,9 @@ typedef struct fbf_search_opts {
                              candidate-list memory (about 0.25 MB per query at the BF16
                              defaults on 10 M rows); results do not depend on it.  Captured
                              searches (fbf_search_graph_create) are not split. */
+    uint32_t metas_on_device; /* fbf_search only: 1 -> metas is DEVICE memory [nq][4]; the
+                                 ranges come from the device router (fbf_route) and the host
+                                 plans from them.  Results equal those of host metas. */
 } fbf_search_opts;
 
 /* Create a device-resident dataset on the current HIP device from host
@@ -123,6 +126,20 @@ fbf_status fbf_filter_count(const fbf_dataset *ds, const float *meta, uint64_t *
 fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, const float *metas,
                       uint64_t nq, uint32_t k, const fbf_search_opts *opts,
                       uint32_t *out_ids_dev, float *out_dists_dev, void *stream);
+
+/* The router's range step on the device (hybrid_graph.cpp:168-201 and the
+ * query order of ReadSortedQuery, utils.h:514-547) for a batch whose metas
+ * are in device memory [nq][4]: ranges_dev[q] = {layout, s, e, bf} -- the
+ * query's contiguous range [s, e) of sorted layout `layout` (0 = (label,
+ * timestamp), 1 = timestamp; type 0: all of layout 1), as fbf_filter_count
+ * gives it, and bf = 1 when the router sends it to brute force: selectivity
+ * (e - s) / n below bf_thr, or type 3 and below bf_thr3 (0 -> 0.045 and 0.08,
+ * baseline.cpp:77, hybrid_graph.h:79; type 0 has selectivity 1).  An unknown
+ * filter type gives {0xFFFFFFFF, 0, 0, 0}.  order_dev (may be NULL) receives
+ * the query indices in ascending (type, v, l, r, index) order -- the
+ * router's batch order (sorted_ids).  Device arrays; waits for its work. */
+fbf_status fbf_route(fbf_dataset *ds, const float *metas_dev, uint64_t nq, float bf_thr, float bf_thr3,
+                     uint32_t *ranges_dev, uint32_t *order_dev, void *stream);
 
 /* Host-memory convenience form of fbf_search (copies in and out, waits). */
 fbf_status fbf_search_host(fbf_dataset *ds, const float *queries, const float *metas,

@@ -163,6 +163,12 @@ struct fbf_dataset {
     // host metadata for the planner
     std::vector<int32_t> lab_b;
     std::vector<float> ts_b, ts_t;
+    // the same sorted metadata on the device (the device router, fbf_route)
+    int32_t *d_lab_b = nullptr;
+    float *d_ts_b = nullptr, *d_ts_t = nullptr;
+    // the device router's buffers (one routing at a time: mu_route)
+    std::mutex mu_route;
+    DevBuf r_keys, r_vals, r_temp, r_ranges;
     // device data
     float *base_rows = nullptr;  // [n][d] original order
     LayoutDev lay[2];
@@ -293,6 +299,12 @@ static fbf_status dataset_build(fbf_dataset *ds, const float *labels, const floa
         ds->ts_b[i] = timestamps[pb[i]];
         ds->ts_t[i] = timestamps[pt[i]];
     }
+    HIP_TRY(hipMalloc(&ds->d_lab_b, n * sizeof(int32_t)));
+    HIP_TRY(hipMalloc(&ds->d_ts_b, n * sizeof(float)));
+    HIP_TRY(hipMalloc(&ds->d_ts_t, n * sizeof(float)));
+    HIP_TRY(hipMemcpy(ds->d_lab_b, ds->lab_b.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ds->d_ts_b, ds->ts_b.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ds->d_ts_t, ds->ts_t.data(), n * sizeof(float), hipMemcpyHostToDevice));
     const uint64_t n_pad = ds->n_pad;
     std::vector<uint32_t> *perms[2] = {&pb, &pt};
     for (int L = 0; L < 2; ++L) {
@@ -488,6 +500,9 @@ extern "C" fbf_status fbf_dataset_destroy(fbf_dataset *ds) {
     for (float *p : {ds->sq8_mi, ds->sq8_dif, ds->sq8_w})
         if (p) (void)hipFree(p);
     if (ds->tb_codes) (void)hipFree(ds->tb_codes);
+    for (void *p : {(void *)ds->d_lab_b, (void *)ds->d_ts_b, (void *)ds->d_ts_t})
+        if (p) (void)hipFree(p);
+    for (DevBuf *b : {&ds->r_keys, &ds->r_vals, &ds->r_temp, &ds->r_ranges}) b->release();
     ds->scr.release();
     for (auto &ev : ds->events) {
         if (ev.a) hipEventDestroy(ev.a);
@@ -802,8 +817,8 @@ struct GraphOut {
 static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, const float *queries_dev,
                               const float *metas, uint64_t nq, uint32_t k, const fbf_search_opts *opts,
                               uint32_t *out_ids_dev, float *out_dists_dev, hipStream_t st, GraphOut *cap,
-                              uint32_t *cert = nullptr) {
-    if (!ds || !metas || (nq && (!queries_dev || !out_ids_dev || !out_dists_dev)))
+                              uint32_t *cert = nullptr, const QRange *ranges_in = nullptr) {
+    if (!ds || (!metas && !ranges_in) || (nq && (!queries_dev || !out_ids_dev || !out_dists_dev)))
         return fail(FBF_ERR_INVALID_ARG, "null argument");
     if (k == 0 || k > 512) return fail(FBF_ERR_INVALID_ARG, "k must be in [1, 512]");
     if (nq == 0) return FBF_OK;
@@ -871,8 +886,13 @@ static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, 
     // neither goes into a captured graph.
     const bool prof = ds->profile && cap == nullptr;
     std::vector<QRange> qr;
-    s = compute_ranges(ds, metas, nq, qr);
-    if (s != FBF_OK) return s;
+    if (ranges_in) {
+        qr.assign(ranges_in, ranges_in + nq);
+        for (uint64_t i = 0; i < nq; ++i) qr[i].qi = (uint32_t)i;
+    } else {
+        s = compute_ranges(ds, metas, nq, qr);
+        if (s != FBF_OK) return s;
+    }
     // Approximate modes: a prefix pass over the first 1/prepass_div of every
     // large range keeps only the kpre = margin * k' / prepass_div best keys of
     // that prefix; its kpre-th key (tau0) estimates the k'-th key of the whole
@@ -1160,10 +1180,10 @@ static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, 
 // distances, (distance, id) order), so the whole output does.
 static fbf_status search_exact_bf16(fbf_dataset *ds, const float *queries_dev, const float *metas, uint64_t nq,
                                     uint32_t k, const fbf_search_opts &o, uint32_t *out_ids_dev,
-                                    float *out_dists_dev, hipStream_t st) {
+                                    float *out_dists_dev, hipStream_t st, const QRange *ranges_in = nullptr) {
     fbf_status s = mode_check(ds, FBF_MODE_EXACT_BF16);
     if (s != FBF_OK) return s;
-    if (!metas || (nq && (!queries_dev || !out_ids_dev || !out_dists_dev)))
+    if ((!metas && !ranges_in) || (nq && (!queries_dev || !out_ids_dev || !out_dists_dev)))
         return fail(FBF_ERR_INVALID_ARG, "null argument");
     if (k == 0 || k > 512) return fail(FBF_ERR_INVALID_ARG, "k must be in [1, 512]");
     std::lock_guard<std::mutex> lk(ds->mu_cert);
@@ -1185,7 +1205,7 @@ static fbf_status search_exact_bf16(fbf_dataset *ds, const float *queries_dev, c
     Scratch &S = ds->scr;
     HIP_TRY(S.d_cert.reserve(nq * sizeof(uint32_t)));
     s = search_impl(ds, S, true, queries_dev, metas, nq, k, &ob, out_ids_dev, out_dists_dev, st, nullptr,
-                    S.d_cert.as<uint32_t>());
+                    S.d_cert.as<uint32_t>(), ranges_in);
     if (s != FBF_OK) return s;
     // The search statistics stay those of the BF16 search (the fallback is
     // counted by last_fallbacks).
@@ -1211,14 +1231,22 @@ static fbf_status search_exact_bf16(fbf_dataset *ds, const float *queries_dev, c
         HIP_TRY(S.d_sub_q.reserve(m * kDim * sizeof(float)));
         HIP_TRY(S.d_sub_ids.reserve(m * k * sizeof(uint32_t)));
         HIP_TRY(S.d_sub_d.reserve(m * k * sizeof(float)));
-        std::vector<float> msub(m * 4);
-        for (uint64_t j = 0; j < m; ++j) std::memcpy(&msub[4 * j], metas + 4 * (uint64_t)sub[j], 4 * sizeof(float));
+        std::vector<float> msub;
+        std::vector<QRange> rsub;
+        if (ranges_in) {
+            rsub.resize(m);
+            for (uint64_t j = 0; j < m; ++j) rsub[j] = ranges_in[sub[j]];
+        } else {
+            msub.resize(m * 4);
+            for (uint64_t j = 0; j < m; ++j) std::memcpy(&msub[4 * j], metas + 4 * (uint64_t)sub[j], 4 * sizeof(float));
+        }
         HIP_TRY(hipMemcpyAsync(S.d_sub_idx.p, sub.data(), m * sizeof(uint32_t), hipMemcpyHostToDevice, st));
         HIP_TRY(launch_gather_rows(queries_dev, S.d_sub_idx.as<uint32_t>(), m, S.d_sub_q.as<float>(), st));
         fbf_search_opts oe{};
         oe.mode = FBF_MODE_EXACT_F32;
-        s = search_impl(ds, S, true, S.d_sub_q.as<float>(), msub.data(), m, k, &oe, S.d_sub_ids.as<uint32_t>(),
-                        S.d_sub_d.as<float>(), st, nullptr);
+        s = search_impl(ds, S, true, S.d_sub_q.as<float>(), ranges_in ? nullptr : msub.data(), m, k, &oe,
+                        S.d_sub_ids.as<uint32_t>(), S.d_sub_d.as<float>(), st, nullptr, nullptr,
+                        ranges_in ? rsub.data() : nullptr);
         if (s != FBF_OK) return s;
         HIP_TRY(launch_scatter_results(S.d_sub_ids.as<uint32_t>(), S.d_sub_d.as<float>(), S.d_sub_idx.as<uint32_t>(),
                                        m, k, out_ids_dev, out_dists_dev, st));
@@ -1235,10 +1263,93 @@ static fbf_status search_exact_bf16(fbf_dataset *ds, const float *queries_dev, c
     return FBF_OK;
 }
 
+// The device router for a batch: ranges (and, when order is not null, the
+// ReadSortedQuery order) on stream st into ds->r_ranges; the caller holds
+// ds->mu_route.
+static fbf_status route_device(fbf_dataset *ds, const float *metas_dev, uint64_t nq, float bf_thr, float bf_thr3,
+                               uint32_t *order_dev, hipStream_t st) {
+    HIP_TRY(ds->r_ranges.reserve(std::max<uint64_t>(1, nq) * 4 * sizeof(uint32_t)));
+    HIP_TRY(ds->r_keys.reserve(std::max<uint64_t>(1, nq) * 4 * sizeof(uint64_t)));
+    HIP_TRY(ds->r_vals.reserve(std::max<uint64_t>(1, nq) * 2 * sizeof(uint32_t)));
+    const size_t tb = route_temp_bytes(nq);
+    HIP_TRY(ds->r_temp.reserve(std::max<size_t>(1, tb)));
+    HIP_TRY(launch_route(metas_dev, nq, ds->d_lab_b, ds->d_ts_b, ds->d_ts_t, ds->n, (float)ds->n, bf_thr, bf_thr3,
+                         ds->r_ranges.as<uint32_t>(), order_dev, ds->r_keys.as<uint64_t>(), ds->r_vals.as<uint32_t>(),
+                         ds->r_temp.p, tb, st));
+    return FBF_OK;
+}
+
+// Ranges of a batch whose metas are in device memory: routed on the device,
+// copied to the host for the planner.
+static fbf_status device_ranges(fbf_dataset *ds, const float *metas_dev, uint64_t nq, hipStream_t st,
+                                std::vector<QRange> &qr) {
+    std::vector<uint32_t> h(4 * nq);
+    {
+        std::lock_guard<std::mutex> lk(ds->mu_route);
+        HIP_TRY(hipSetDevice(ds->device));
+        fbf_status s = route_device(ds, metas_dev, nq, 0.045f, 0.08f, nullptr, st);
+        if (s != FBF_OK) return s;
+        HIP_TRY(hipMemcpyAsync(h.data(), ds->r_ranges.p, 4 * nq * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+    qr.resize(nq);
+    for (uint64_t i = 0; i < nq; ++i) {
+        if (h[4 * i] == 0xFFFFFFFFu) return fail(FBF_ERR_INVALID_ARG, "query " + std::to_string(i) + ": bad filter type");
+        qr[i] = QRange{h[4 * i], h[4 * i + 1], h[4 * i + 2], (uint32_t)i};
+    }
+    return FBF_OK;
+}
+
+extern "C" fbf_status fbf_route(fbf_dataset *ds, const float *metas_dev, uint64_t nq, float bf_thr, float bf_thr3,
+                                uint32_t *ranges_dev, uint32_t *order_dev, void *stream) {
+    if (!ds || (nq && (!metas_dev || !ranges_dev))) return fail(FBF_ERR_INVALID_ARG, "null argument");
+    if (nq >= 0xFFFFFFFFull) return fail(FBF_ERR_INVALID_ARG, "batch too large");
+    if (nq == 0) return FBF_OK;
+    if (bf_thr == 0.f) bf_thr = 0.045f;   // baseline.cpp:77
+    if (bf_thr3 == 0.f) bf_thr3 = 0.08f;  // hybrid_graph.h:79
+    hipStream_t st = (hipStream_t)stream;
+    std::lock_guard<std::mutex> lk(ds->mu_route);
+    HIP_TRY(hipSetDevice(ds->device));
+    fbf_status s = route_device(ds, metas_dev, nq, bf_thr, bf_thr3, order_dev, st);
+    if (s != FBF_OK) return s;
+    HIP_TRY(hipMemcpyAsync(ranges_dev, ds->r_ranges.p, 4 * nq * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+    // the router's buffers are reused by the next call: let this one finish
+    HIP_TRY(hipStreamSynchronize(st));
+    return FBF_OK;
+}
+
 extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, const float *metas,
                                  uint64_t nq, uint32_t k, const fbf_search_opts *opts,
                                  uint32_t *out_ids_dev, float *out_dists_dev, void *stream) {
     if (!ds) return fail(FBF_ERR_INVALID_ARG, "null argument");
+    if (opts && opts->metas_on_device) {
+        // metas in device memory: the device router gives the ranges, the
+        // host plans from them (searches in sub-batches as below).
+        if (!metas || (nq && (!queries_dev || !out_ids_dev || !out_dists_dev)))
+            return fail(FBF_ERR_INVALID_ARG, "null argument");
+        if (k == 0 || k > 512) return fail(FBF_ERR_INVALID_ARG, "k must be in [1, 512]");
+        if (nq == 0) return FBF_OK;
+        std::vector<QRange> qr;
+        fbf_status s = device_ranges(ds, metas, nq, (hipStream_t)stream, qr);
+        if (s != FBF_OK) return s;
+        const uint64_t max_batch = opts->max_batch ? opts->max_batch : 131072u;
+        for (uint64_t i0 = 0; i0 < nq; i0 += max_batch) {
+            const uint64_t m = std::min<uint64_t>(max_batch, nq - i0);
+            if (opts->mode == FBF_MODE_EXACT_BF16) {
+                s = search_exact_bf16(ds, queries_dev + i0 * ds->d, nullptr, m, k, *opts, out_ids_dev + i0 * k,
+                                      out_dists_dev + i0 * k, (hipStream_t)stream, qr.data() + i0);
+            } else {
+                {
+                    std::lock_guard<std::mutex> lock(ds->mu);
+                    ds->last_fallbacks = 0;
+                }
+                s = search_impl(ds, ds->scr, true, queries_dev + i0 * ds->d, nullptr, m, k, opts, out_ids_dev + i0 * k,
+                                out_dists_dev + i0 * k, (hipStream_t)stream, nullptr, nullptr, qr.data() + i0);
+            }
+            if (s != FBF_OK) return s;
+        }
+        return FBF_OK;
+    }
     // Very large batches run as consecutive sub-batches (bounded list memory).
     const uint64_t max_batch = opts && opts->max_batch ? opts->max_batch : 131072u;
     if (nq > max_batch) {

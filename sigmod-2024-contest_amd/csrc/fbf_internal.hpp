@@ -199,6 +199,14 @@ hipError_t launch_pool_keys(const uint32_t *ids, const uint32_t *dbits, const Sl
 hipError_t launch_ext_keys(const uint32_t *pos, const float *dist, const uint32_t *ext_slot, uint64_t nq, uint32_t n_ext,
                            uint64_t n_pos, uint32_t stride, uint64_t *lists, uint32_t *counts, hipStream_t st);
 
+// Device router: per-query ranges [layout, s, e, bf] and (order != null) the
+// batch order of ReadSortedQuery.  keys: 4 nq u64, vals: 2 nq u32, temp:
+// route_temp_bytes(nq) bytes.
+size_t route_temp_bytes(uint64_t nq);
+hipError_t launch_route(const float *metas, uint64_t nq, const int32_t *lab_b, const float *ts_b, const float *ts_t,
+                        uint64_t n, float nb, float bf_thr, float bf_thr3, uint32_t *ranges, uint32_t *order,
+                        uint64_t *keys, uint32_t *vals, void *temp, size_t temp_bytes, hipStream_t st);
+
 hipError_t launch_scan_f32(const ScanArgs &a, int B, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st);
 hipError_t launch_merge(const MergeArgs &a, hipStream_t st);

@@ -41,7 +41,7 @@ EXPORTS = [
     "fbf_profile_enable", "fbf_profile_collect", "fbf_last_search_stats", "fbf_last_search_rescans",
     "fbf_last_error", "fbf_version", "fbf_search_graph_create", "fbf_search_graph_launch",
     "fbf_search_graph_destroy", "fbf_last_search_fallbacks", "fbf_dataset_layout_rows",
-    "fbf_last_search_pass_pairs", "fbf_bruteforce_batch_noalloc", "fbf_search_subinterval",
+    "fbf_last_search_pass_pairs", "fbf_bruteforce_batch_noalloc", "fbf_search_subinterval", "fbf_route",
 ]
 
 
@@ -57,7 +57,7 @@ class SubintervalOpts(C.Structure):
 class SearchOpts(C.Structure):
     _fields_ = [("mode", C.c_uint32), ("kprime", C.c_uint32), ("chunk_rows", C.c_uint32),
                 ("group_queries", C.c_uint32), ("persistent", C.c_uint32), ("prepass_div", C.c_uint32),
-                ("prepass_margin", C.c_uint32), ("max_batch", C.c_uint32)]
+                ("prepass_margin", C.c_uint32), ("max_batch", C.c_uint32), ("metas_on_device", C.c_uint32)]
 
 
 _lib = None
@@ -81,6 +81,7 @@ def lib() -> C.CDLL:
     L.fbf_filter_count.argtypes = [_P, _P, C.POINTER(C.c_uint64), C.POINTER(C.c_float)]
     L.fbf_search.argtypes = [_P, _P, _P, C.c_uint64, C.c_uint32, C.POINTER(SearchOpts), _P, _P, _P]
     L.fbf_search_host.argtypes = [_P, _P, _P, C.c_uint64, C.c_uint32, C.POINTER(SearchOpts), _P, _P]
+    L.fbf_route.argtypes = [_P, _P, C.c_uint64, C.c_float, C.c_float, _P, _P, _P]
     L.fbf_dataset_quant_params.argtypes = [_P, _P, _P, _P]
     L.fbf_register_codes.argtypes = [_P, C.c_uint64]
     L.fbf_unregister_codes.argtypes = [_P]
@@ -196,7 +197,11 @@ class Dataset:
         on the queries' device; asynchronous on ``stream`` (torch stream or None
         for torch's current stream)."""
         import torch
-        metas = np.ascontiguousarray(metas, np.float32)
+        on_dev = isinstance(metas, torch.Tensor)
+        if on_dev:  # device metas: routed on the device (fbf_route)
+            assert metas.is_cuda and metas.dtype == torch.float32 and metas.is_contiguous()
+        else:
+            metas = np.ascontiguousarray(metas, np.float32)
         nq = metas.shape[0]
         assert queries.shape == (nq, self.d) and queries.dtype == torch.float32
         if out is None:
@@ -207,9 +212,24 @@ class Dataset:
         st = (stream or torch.cuda.current_stream(queries.device)).cuda_stream
         o = self._opts(mode, kprime, chunk_rows, group_queries, persistent, prepass_div, prepass_margin,
                        max_batch)
-        _check(lib().fbf_search(self.h, _P(_dev_ptr(queries)), _np_ptr(metas), nq, k, C.byref(o),
+        o.metas_on_device = int(on_dev)
+        mp = _P(_dev_ptr(metas)) if on_dev else _np_ptr(metas)
+        _check(lib().fbf_search(self.h, _P(_dev_ptr(queries)), mp, nq, k, C.byref(o),
                                 _P(_dev_ptr(ids)), _P(_dev_ptr(dists)), _P(st)), "fbf_search")
         return ids, dists
+
+    def route(self, metas_dev, bf_thr: float = 0.0, bf_thr3: float = 0.0, order: bool = True, stream=None):
+        """Device router (fbf_route): metas torch CUDA float32 [nq, 4] ->
+        (ranges int32 [nq, 4] = layout, s, e, bf; order int32 [nq] or None)."""
+        import torch
+        assert metas_dev.is_cuda and metas_dev.dtype == torch.float32 and metas_dev.is_contiguous()
+        nq = metas_dev.shape[0]
+        ranges = torch.empty((nq, 4), dtype=torch.int32, device=metas_dev.device)
+        ordr = torch.empty(nq, dtype=torch.int32, device=metas_dev.device) if order else None
+        st = (stream or torch.cuda.current_stream(metas_dev.device)).cuda_stream
+        _check(lib().fbf_route(self.h, _P(_dev_ptr(metas_dev)), nq, bf_thr, bf_thr3, _P(_dev_ptr(ranges)),
+                               _P(_dev_ptr(ordr)) if order else None, _P(st)), "fbf_route")
+        return ranges, ordr
 
     def search_host(self, queries: np.ndarray, metas: np.ndarray, k: int = 100, mode="exact",
                     kprime: int = 0, chunk_rows: int = 0, group_queries: int = 0,

@@ -173,3 +173,37 @@ def test_c1_subinterval_graph_routed_type2(orc, c1):
     assert np.array_equal(types[s], ot)
     assert np.array_equal(ids[s], oid)
     assert np.array_equal(d[s], od)
+
+
+def _router_reference(m, counts, n):
+    """The routing rule (hybrid_graph.cpp:200-201, selectivity in float32 as
+    the reference computes it) and ReadSortedQuery's order (utils.h:514-547,
+    the query index as the last key)."""
+    sel = (counts.astype(np.float32) / np.float32(n)).astype(np.float32)
+    sel[m[:, 0] == 0] = np.float32(1.0)
+    bf = (sel < np.float32(0.045)) | ((m[:, 0] == 3) & (sel < np.float32(0.08)))
+    order = np.lexsort((np.arange(m.shape[0]), m[:, 3], m[:, 2], m[:, 1].astype(np.int32), m[:, 0].astype(np.uint32)))
+    return bf, order
+
+
+def test_c1_device_router(orc, c1):
+    import torch
+    m = c1["metas"]
+    _say("case: N=10M device router over 100k queries vs the oracle's ranges and the routing rule")
+    md = torch.from_numpy(m).cuda()
+    ranges, order = c1["ds"].route(md)
+    torch.cuda.synchronize()
+    r = ranges.cpu().numpy().view(np.uint32)
+    ref = c1["L"].ranges(m)
+    assert np.array_equal(r[:, :3].astype(np.int64), ref)
+    bf, ref_order = _router_reference(m, c1["counts"], c1["vecs"].shape[0])
+    assert np.array_equal(r[:, 3].astype(bool), bf)
+    assert np.array_equal(order.cpu().numpy(), ref_order.astype(np.int32))
+    _say(f"  {int(bf.sum())} queries routed to brute force")
+    _say("case: N=10M bf16 and exact_bf16 searches with device metas vs host metas")
+    for mode in ("bf16", "exact_bf16"):
+        a = _search(c1, mode=mode)
+        ids, d = c1["ds"].search(c1["qd"], md, k=K, mode=mode)
+        torch.cuda.synchronize()
+        assert np.array_equal(ids.cpu().numpy().view(np.uint32), a[0]), mode
+        assert np.array_equal(d.cpu().numpy(), a[1]), mode
