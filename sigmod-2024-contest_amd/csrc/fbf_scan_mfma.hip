@@ -581,13 +581,12 @@ constexpr uint32_t kShareTiles = 512;  // tiles between threshold exchanges with
 // Bit j (score j of the lane: row r0 + 8 (j / 4) + (j % 4), r0 = tile row +
 // 4 * half) set when that row lies in [qs, qe).
 __device__ __attribute__((always_inline)) inline uint32_t range_bits(uint32_t t, uint32_t half, uint32_t qs, uint32_t qe) {
-    uint32_t m = 0;
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        const uint32_t row = t + 8 * (j / 4) + 4 * half + (j % 4);
-        m |= (row >= qs && row < qe ? 1u : 0u) << j;
-    }
-    return m;
+    const uint32_t lo = qs <= t ? 0u : min(qs - t, 32u);
+    const uint32_t hi = qe <= t ? 0u : min(qe - t, 32u);
+    const uint32_t below_hi = hi >= 32 ? 0xFFFFFFFFu : ((1u << hi) - 1u);
+    const uint32_t below_lo = lo >= 32 ? 0xFFFFFFFFu : ((1u << lo) - 1u);
+    const uint32_t sh = (below_hi & ~below_lo) >> (4 * half);
+    return (sh & 0xFu) | ((sh >> 4) & 0xF0u) | ((sh >> 8) & 0xF00u) | ((sh >> 12) & 0xF000u);
 }
 
 // Per-lane select: lanes in m take b, the others a.  Written as the
@@ -699,6 +698,27 @@ __device__ __attribute__((always_inline)) void scan_mfma_item(const ScanMfmaArgs
         cnt[b] = 0;
         mylist[b] = list_of(j) + half * H;
     }
+    // Per block: does it hold a query of this item (blk_on), and the tiles
+    // [fl, fh) of the item that every one of its queries covers whole -- in
+    // those the block needs no per-lane range test (wave-uniform bounds).
+    bool blk_on[kQB];
+    uint32_t fl[kQB], fh[kQB];
+#pragma unroll
+    for (int b = 0; b < kQB; ++b) {
+        blk_on[b] = __ballot(qv[b]) != 0;
+        uint32_t smax = qv[b] ? qs[b] : 0u, emin = qv[b] ? qe[b] : 0xFFFFFFFFu;
+#pragma unroll
+        for (int off = 16; off > 0; off >>= 1) {
+            smax = max(smax, (uint32_t)__shfl_xor((int)smax, off));
+            emin = min(emin, (uint32_t)__shfl_xor((int)emin, off));
+        }
+        smax = __builtin_amdgcn_readfirstlane(smax);
+        emin = __builtin_amdgcn_readfirstlane(emin);
+        const uint32_t lo = smax <= W.row_begin ? 0u : (smax - W.row_begin + kMTile - 1) / kMTile;
+        const uint32_t hi = emin <= W.row_begin ? 0u : (emin - W.row_begin) / kMTile;
+        fl[b] = lo;
+        fh[b] = hi > lo ? hi : lo;
+    }
 
     // Rows stream through four LDS tile buffers in pairs of tiles: while the
     // matrix cores work on a pair, the direct loads of the next pair are in
@@ -729,10 +749,14 @@ __device__ __attribute__((always_inline)) void scan_mfma_item(const ScanMfmaArgs
         const bool live = m < ntile;                                                                  \
         const uint32_t t = W.row_begin + m * kMTile;                                                  \
         bool hit_any = false;                                                                         \
+        bool inner[kQB];                                                                              \
         uint64_t hit[kQB];                                                                            \
 _Pragma("unroll")                                                                                     \
         for (int b = 0; b < kQB; ++b) {                                                               \
-            hit[b] = __ballot(live && qv[b] && t < qe[b] && t + kMTile > qs[b]);                      \
+/* Interior tiles (every query of the block covers the tile): a scalar  */                            \
+/* test; elsewhere the lanes test their own ranges.                      */                           \
+            inner[b] = live && blk_on[b] && m >= fl[b] && m < fh[b];                                  \
+            hit[b] = inner[b] ? ~0ull : __ballot(live && qv[b] && t < qe[b] && t + kMTile > qs[b]);   \
             hit_any |= hit[b] != 0;                                                                   \
         }                                                                                             \
 /* FP8 row scales, FP8 / SYM row norms: loaded before the tile's      */                            \
@@ -776,8 +800,8 @@ _Pragma("unroll")                                                               
                 float mn = sc[0];                                                                     \
 _Pragma("unroll")                                                                                     \
                 for (int j = 1; j < 16; ++j) mn = fminf(mn, sc[j]);                                   \
-                const bool full = t >= qs[b] && t + kMTile <= qe[b];                                  \
-                const float th = (t < qe[b] && t + kMTile > qs[b]) ? tauf[b] : -__builtin_inff();     \
+                const float th = inner[b] ? tauf[b]                                                   \
+                                 : (t < qe[b] && t + kMTile > qs[b]) ? tauf[b] : -__builtin_inff();   \
                 if (__ballot(mn <= th) == 0) continue;                                                \
 /* Rare path: a per-lane bit mask of the scores at or below the lane's  */                           \
 /* threshold (inside its query's range on a boundary tile); each lane    */                           \
@@ -789,7 +813,10 @@ _Pragma("unroll")                                                               
                 uint32_t pm = 0;                                                                      \
 _Pragma("unroll")                                                                                     \
                 for (int j = 0; j < 16; ++j) pm |= (sc[j] <= th ? 1u : 0u) << j;                      \
-                if (__ballot(pm != 0 && !full)) pm &= range_bits(t, half, qs[b], qe[b]);              \
+                if (!inner[b]) {                                                                      \
+                    const bool full = t >= qs[b] && t + kMTile <= qe[b];                              \
+                    if (__ballot(pm != 0 && !full)) pm &= range_bits(t, half, qs[b], qe[b]);          \
+                }                                                                                     \
                 append_bits(pm, sc, t + 4 * half, lay_bit, mylist[b], cnt[b]);                        \
             }                                                                                         \
         }                                                                                             \
