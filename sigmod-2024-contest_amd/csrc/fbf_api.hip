@@ -115,7 +115,7 @@ struct ProfEvent {
 // Per-call scratch: the uploaded plan, the per-(item, query) candidate lists
 // and the pinned staging buffer the plan is copied through.
 struct Scratch {
-    DevBuf d_items, d_iq, d_slot_off, d_lists, d_counts, d_queue, d_q, d_out_ids, d_out_d, d_gtau,
+    DevBuf d_items, d_iq, d_slot_off, d_lists, d_counts, d_queue, d_q, d_qf, d_out_ids, d_out_d, d_gtau,
         d_tau0, d_flags, d_pre_end;
     // FBF_MODE_EXACT_BF16: per-query certificates and the uncertified subset.
     DevBuf d_cert, d_sub_idx, d_sub_q, d_sub_ids, d_sub_d;
@@ -985,6 +985,7 @@ static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, 
         if (!S.ev_nflag) HIP_TRY(hipEventCreateWithFlags(&S.ev_nflag, hipEventDisableTiming));
     }
     if (o.mode == FBF_MODE_SQ8SYM) HIP_TRY(S.d_q.reserve(nq * 112));
+    if (mfma) HIP_TRY(S.d_qf.reserve(nq * kQFragU4 * sizeof(uint4)));
     ds->last_seeded = false;
     if (cap) {
         // The plan is on the device once the stream drains; the launches
@@ -1032,16 +1033,14 @@ static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, 
     if (mfma) {
         const int variant = o.mode == FBF_MODE_BF16 ? 0 : o.mode == FBF_MODE_FP8 ? 1 : o.mode == FBF_MODE_SQ8 ? 2 : 3;
         ScanMfmaArgs a{};
-        if (variant == 3) {
-            // Query codes with the dataset's scale (SQ8SymmetricQuantizer::encode).
-            a.qcodes = S.d_q.as<int8_t>();
-            HIP_TRY(launch_sym_encode(queries_dev, nullptr, nq, nq, ds->alpha, S.d_q.as<int8_t>(), st));
-        }
+        // Query codes with the dataset's scale (SQ8SymmetricQuantizer::encode).
+        if (variant == 3) HIP_TRY(launch_sym_encode(queries_dev, nullptr, nq, nq, ds->alpha, S.d_q.as<int8_t>(), st));
+        HIP_TRY(launch_qfrag(variant, queries_dev, ds->sq8_w, variant == 3 ? S.d_q.as<int8_t>() : nullptr, nq,
+                             S.d_qf.as<uint4>(), st));
         a.items = S.d_items.as<WorkItem>();
         a.n_items = P.items.size();
         a.iq = S.d_iq.as<ItemQuery>();
-        a.queries = queries_dev;
-        a.sq8_w = ds->sq8_w;
+        a.qfrag = S.d_qf.as<uint4>();
         for (int L = 0; L < 2; ++L) {
             const LayoutDev &D = ds->lay[L];
             a.lay[L].ids = D.ids;

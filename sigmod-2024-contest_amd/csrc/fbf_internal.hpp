@@ -38,6 +38,16 @@ __device__ __host__ inline uint64_t list_slot(const ItemQuery &e, const WorkItem
     return (uint64_t)e.slot_base + W.chunk;
 }
 
+// Count word of one list slot: the slot holds keys [0, c0) and [h, h + c1)
+// (a list kept as two halves that grew separately; h = 0 and c1 = 0 for a
+// contiguous list of c0 keys).  c0, c1 < 2048; h a multiple of 64 below 2^16.
+__device__ __host__ inline uint32_t list_count_word(uint32_t c0, uint32_t c1, uint32_t h) {
+    return c0 | (c1 << 11) | ((h >> 6) << 22);
+}
+__device__ __host__ inline uint32_t list_count_c0(uint32_t w) { return w & 0x7FFu; }
+__device__ __host__ inline uint32_t list_count_c1(uint32_t w) { return (w >> 11) & 0x7FFu; }
+__device__ __host__ inline uint32_t list_count_h(uint32_t w) { return (w >> 22) << 6; }
+
 constexpr int kMaxGroup = 16;    // queries per work item
 
 // (distance, id) key: distance bits in the high word.  For the non-negative
@@ -124,9 +134,7 @@ struct ScanMfmaArgs {
     const WorkItem *items;
     uint64_t n_items;
     const ItemQuery *iq;
-    const float *queries;    // [nq][kDim] FP32
-    const int8_t *qcodes;    // [nq][112] symmetric int8 query codes (SYM)
-    const float *sq8_w;      // [kDim] dif_k / 255 (SQ8)
+    const uint4 *qfrag;      // [nq][kQFragU4] B-operand fragments per query (launch_qfrag)
     LayoutMfma lay[2];
     uint64_t *lists;
     uint32_t *counts;
@@ -225,6 +233,11 @@ hipError_t launch_scan_sq8sym(const ScanSymArgs &a, int B, hipStream_t st);
 // variant 3 reads the layout's symmetric codes from sq8_codes and writes the
 // code norms to norms.
 hipError_t launch_scan_mfma(int variant, const ScanMfmaArgs &a, int B, int nw, hipStream_t st);
+// B-operand fragments of a batch's queries for the MFMA scans, kQFragU4
+// uint4 per query (variant 3 reads the queries' symmetric int8 codes).
+constexpr int kQFragU4 = 16;
+hipError_t launch_qfrag(int variant, const float *queries, const float *sq8_w, const int8_t *qcodes, uint64_t nq,
+                        uint4 *out, hipStream_t st);
 // Resolve (and cache) the persistent grid of a scan instantiation ahead of a
 // stream capture.
 void warm_scan_mfma(int variant, int B, int nw);

@@ -229,12 +229,13 @@ __global__ __launch_bounds__(64) void merge_kernel(MergeArgs a) {
 #pragma unroll 1
     for (uint64_t si = s0; si < s1; ++si) {
         const uint64_t slot = si;
-        const uint32_t c = a.counts[slot];
+        const uint32_t w = a.counts[slot];
+        const uint32_t c0 = list_count_c0(w), c = c0 + list_count_c1(w), h = list_count_h(w);
         const uint64_t *L = a.lists + slot * (uint64_t)a.list_cap;
 #pragma unroll 1
         for (uint32_t i = 0; i < c; i += 64) {
             const uint32_t ii = i + lane;
-            const uint64_t key = ii < c ? L[ii] : kKeyMax;
+            const uint64_t key = ii < c ? L[ii < c0 ? ii : h + (ii - c0)] : kKeyMax;
             offer_lds<B>(list, cnt, tau, a.kprime, key < tau, key);
         }
     }

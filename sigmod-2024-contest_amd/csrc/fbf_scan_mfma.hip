@@ -428,68 +428,110 @@ __device__ inline void frag_from_lds(const uint4 *sA, uint32_t lane, RowFrag<VAR
     }
 }
 
-// Query fragment of this lane's column (query qi of the batch), or zeros.
-template <int VAR>
-__device__ inline void load_query(const ScanMfmaArgs &a, bool valid, uint32_t qi, uint32_t lane, QFrag<VAR> &f) {
-    const uint32_t half = lane >> 5;
-    const float *q = a.queries + (uint64_t)qi * kDim;
-    if constexpr (VAR == 0 || VAR == 2) {
-#pragma unroll
+// B-operand fragments of one query, built once per search (kQFragU4 uint4
+// per query): half h (lanes 32 h .. 32 h + 31 of the MFMA) holds
+//   BF16 / SQ8: u4[7 h + s], s < 7: bf16 of k = 16 s + 8 h + e (e < 8) --
+//               -2 q_k (SQ8: -2 q_k dif_k / 255), 1.0 in the two |x|^2 slots
+//               k = 100, 101, zero beyond;
+//   FP8:        u4[4 h + 2 s + w], s < 2: e4m3 bytes 16 w .. 16 w + 15 of
+//               k = 64 s + 32 h + e, the values -q_k / 2^ex; u4[15].x the
+//               E8M0 scale 2^(ex + 1) in every byte (restores -2 q);
+//   SYM:        u4[4 h + s], s < 4: code bytes k = 32 s + 16 h + e (zero
+//               beyond the 112 code bytes).
+// One thread per (query, half).
+__global__ void qfrag_kernel(int variant, const float *__restrict__ queries, const float *__restrict__ sq8_w,
+                             const int8_t *__restrict__ qcodes, uint64_t nq, uint4 *__restrict__ out) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * nq) return;
+    const uint64_t qi = t >> 1;
+    const uint32_t half = (uint32_t)(t & 1);
+    const float *q = queries + qi * kDim;
+    uint4 *f = out + qi * kQFragU4;
+    if (variant == 0 || variant == 2) {
         for (int s = 0; s < kBfSteps; ++s) {
             uint32_t w[4];
-#pragma unroll
             for (int e2 = 0; e2 < 4; ++e2) {
                 uint16_t h[2];
-#pragma unroll
                 for (int u = 0; u < 2; ++u) {
                     const int k = 16 * s + 8 * half + 2 * e2 + u;
                     float v = 0.f;
-                    if (valid) {
-                        if (k < kDim) v = VAR == 0 ? -2.0f * q[k] : -2.0f * q[k] * a.sq8_w[k];
-                        else if (k == kDim || k == kDim + 1) v = 1.0f;  // |x|^2 slots
-                    }
+                    if (k < kDim) v = variant == 0 ? -2.0f * q[k] : -2.0f * q[k] * sq8_w[k];
+                    else if (k == kDim || k == kDim + 1) v = 1.0f;  // |x|^2 slots
                     h[u] = bf16_bits(v);
                 }
                 w[e2] = h[0] | (uint32_t)h[1] << 16;
             }
-            f.v[s] = bf16x8_from_bits(w[0], w[1], w[2], w[3]);
+            f[7 * half + s] = make_uint4(w[0], w[1], w[2], w[3]);
         }
-    } else if constexpr (VAR == 3) {
-        // The query's symmetric int8 codes (a.qcodes, 112 bytes per query).
-        const int8_t *c = a.qcodes + (uint64_t)qi * kSymBytes;
-#pragma unroll
+    } else if (variant == 3) {
+        const int8_t *c = qcodes + qi * kSymBytes;
         for (int s = 0; s < kI8Steps; ++s) {
             uint32_t w[4] = {0, 0, 0, 0};
-#pragma unroll
             for (int e = 0; e < 16; ++e) {
                 const int k = 32 * s + 16 * half + e;
-                const uint32_t v = (valid && k < kSymBytes) ? (uint32_t)(uint8_t)c[k] : 0u;
+                const uint32_t v = k < kSymBytes ? (uint32_t)(uint8_t)c[k] : 0u;
                 w[e / 4] |= v << (8 * (e % 4));
             }
-            f.v[s] = i32x4{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
+            f[4 * half + s] = make_uint4(w[0], w[1], w[2], w[3]);
         }
     } else {
         float am = 0.f;
-        if (valid)
-            for (int k = 0; k < kDim; ++k) am = fmaxf(am, fabsf(q[k]));
+        for (int k = 0; k < kDim; ++k) am = fmaxf(am, fabsf(q[k]));
         const int ex = fp8_exponent(am);
-#pragma unroll
         for (int s = 0; s < kF8Steps; ++s) {
-            i32x8 w;
-#pragma unroll
+            uint32_t w[8];
             for (int g = 0; g < 8; ++g) {
                 float v[4];
-#pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int k = 64 * s + 32 * half + 4 * g + u;
-                    v[u] = (valid && k < kDim) ? -ldexpf(q[k], -ex) : 0.f;
+                    v[u] = k < kDim ? -ldexpf(q[k], -ex) : 0.f;
                 }
-                w[g] = (int)pack_fp8x4(v[0], v[1], v[2], v[3]);
+                w[g] = pack_fp8x4(v[0], v[1], v[2], v[3]);
             }
-            f.v[s] = w;
+            f[4 * half + 2 * s] = make_uint4(w[0], w[1], w[2], w[3]);
+            f[4 * half + 2 * s + 1] = make_uint4(w[4], w[5], w[6], w[7]);
         }
-        // B values carry -q / 2^ex; the scale 2^(ex+1) restores -2 q.
-        f.scale = (127 + ex + 1) * 0x01010101;
+        if (half == 0) f[15] = make_uint4((uint32_t)(127 + ex + 1) * 0x01010101u, 0u, 0u, 0u);
+    }
+}
+
+hipError_t launch_qfrag(int variant, const float *queries, const float *sq8_w, const int8_t *qcodes, uint64_t nq,
+                        uint4 *out, hipStream_t st) {
+    if (nq == 0) return hipSuccess;
+    if (out == nullptr || (variant == 3 ? qcodes == nullptr : queries == nullptr) || (variant == 2 && sq8_w == nullptr))
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(qfrag_kernel, dim3((uint32_t)((2 * nq + 255) / 256)), dim3(256), 0, st, variant, queries,
+                       sq8_w, qcodes, nq, out);
+    return hipGetLastError();
+}
+
+// This lane's B fragments (its column's query qi, its half), or zeros.
+template <int VAR>
+__device__ inline void load_query(const ScanMfmaArgs &a, bool valid, uint32_t qi, uint32_t lane, QFrag<VAR> &f) {
+    const uint32_t half = lane >> 5;
+    const uint4 *src = a.qfrag + (uint64_t)qi * kQFragU4;
+    const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+    if constexpr (VAR == 0 || VAR == 2) {
+#pragma unroll
+        for (int s = 0; s < kBfSteps; ++s) {
+            const uint4 u = valid ? src[7 * half + s] : z;
+            f.v[s] = bf16x8_from_bits(u.x, u.y, u.z, u.w);
+        }
+    } else if constexpr (VAR == 3) {
+#pragma unroll
+        for (int s = 0; s < kI8Steps; ++s) {
+            const uint4 u = valid ? src[4 * half + s] : z;
+            f.v[s] = i32x4{(int)u.x, (int)u.y, (int)u.z, (int)u.w};
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < kF8Steps; ++s) {
+            const uint4 u0 = valid ? src[4 * half + 2 * s] : z, u1 = valid ? src[4 * half + 2 * s + 1] : z;
+            f.v[s] = i32x8{(int)u0.x, (int)u0.y, (int)u0.z, (int)u0.w, (int)u1.x, (int)u1.y, (int)u1.z, (int)u1.w};
+        }
+        // An invalid lane's values are zero; its scale only needs to be a
+        // valid E8M0 exponent.
+        f.scale = valid ? (int)src[15].x : 127 * 0x01010101;
     }
 }
 
@@ -665,39 +707,53 @@ __device__ __attribute__((always_inline)) void scan_mfma_item(const ScanMfmaArgs
     uint64_t *tau_w = s_tau + q0;
     const LayoutMfma &L = a.lay[W.layout];
     const uint4 *gA = reinterpret_cast<const uint4 *>(L.tiles);
-    auto qi_of = [&](uint32_t j) { return a.iq[W.q_begin + q0 + j].qi; };
-    auto slot_of = [&](uint32_t j) { return list_slot(a.iq[W.q_begin + q0 + j], W); };
-    auto list_of = [&](uint32_t j) { return a.lists + slot_of(j) * (uint64_t)a.list_stride; };
 
-    // Start from the thresholds earlier items (or the prefix pass) published.
-    for (uint32_t j = lane; j < nq; j += 64)
-        tau_w[j] = a.gtau ? __hip_atomic_load(&a.gtau[qi_of(j)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                          : kKeyMax;
-    WaveSync::sync();
+    // Per lane and block: the column's query (j = 32 b + col of the wave's
+    // queries), its range, its list slot and this lane's half of the list.
+    // Lane l holds query l in block l / 32 (col = l % 32, half = l / 32).
     QFrag<VAR> qf[kQB];
     float tauf[kQB];
-    uint32_t qs[kQB], qe[kQB];
+    uint32_t qs[kQB], qe[kQB], qib[kQB];
     bool qv[kQB];
     uint32_t cnt[kQB];          // keys in this lane's half of its query's list
     uint64_t *mylist[kQB];      // this lane's half
+    uint64_t myslot[kQB];
 #pragma unroll
     for (int b = 0; b < kQB; ++b) {
         const uint32_t j = 32 * b + col;
         qv[b] = j < nq;
-        uint32_t qi = 0;
+        qib[b] = 0;
         qs[b] = qe[b] = 0;
+        myslot[b] = 0;
         if (qv[b]) {
             const ItemQuery e = a.iq[W.q_begin + q0 + j];
-            qi = e.qi;
+            qib[b] = e.qi;
             qs[b] = e.s;
             qe[b] = e.e;
-            if (a.qmask && a.qmask[qi] == 0) qv[b] = false;
+            myslot[b] = list_slot(e, W);
+            if (a.qmask && a.qmask[e.qi] == 0) qv[b] = false;
         }
-        load_query<VAR>(a, qv[b], qi, lane, qf[b]);
-        tauf[b] = qv[b] ? tau_as_float(tau_w[j]) : -__builtin_inff();
         cnt[b] = 0;
-        mylist[b] = list_of(j) + half * H;
+        mylist[b] = a.lists + myslot[b] * (uint64_t)a.list_stride + half * H;
     }
+    const uint32_t my_qi = half ? qib[1] : qib[0];  // query j = lane
+    // Every per-query loop below runs j = lane only (nq <= 64).
+    auto qi_of = [&](uint32_t) { return my_qi; };
+    // Base of the list of query j = 32 b + c (held by lane c, half 0).
+    auto list_base = [&](int b, uint32_t c) {
+        const uint64_t p = reinterpret_cast<uint64_t>(mylist[b]);
+        const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)p, c), hi = __builtin_amdgcn_readlane((uint32_t)(p >> 32), c);
+        return reinterpret_cast<uint64_t *>(((uint64_t)hi << 32) | lo);
+    };
+
+    // Start from the thresholds earlier items (or the prefix pass) published.
+    if (lane < nq)
+        tau_w[lane] = a.gtau ? __hip_atomic_load(&a.gtau[my_qi], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : kKeyMax;
+#pragma unroll
+    for (int b = 0; b < kQB; ++b) load_query<VAR>(a, qv[b], qib[b], lane, qf[b]);
+    WaveSync::sync();
+#pragma unroll
+    for (int b = 0; b < kQB; ++b) tauf[b] = qv[b] ? tau_as_float(tau_w[32 * b + col]) : -__builtin_inff();
     // Per block: does it hold a query of this item (blk_on), and the tiles
     // [fl, fh) of the item that every one of its queries covers whole -- in
     // those the block needs no per-lane range test (wave-uniform bounds).
@@ -739,6 +795,41 @@ __device__ __attribute__((always_inline)) void scan_mfma_item(const ScanMfmaArgs
 
     const uint32_t lay_bit = W.layout << 31;
     uint32_t since_share = 0;
+
+    // Threshold test of one block's scores of one tile (first row t), and
+    // the rare append.  One compare per lane: the smallest of its 16 scores
+    // against the lane's threshold, or against -inf when the tile lies wholly
+    // outside the lane's query range (the block runs for the other queries).
+    // Returns whether any lane appended.
+    auto test_block = [&](const f32x16 &acc, const float (&xnv)[16], uint32_t t, int b,
+                          bool inner_b) __attribute__((always_inline)) -> bool {
+        float sc[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) sc[j] = (VAR == 1 || VAR == 3) ? acc[j] + xnv[j] : acc[j];
+        float mn = sc[0];
+#pragma unroll
+        for (int j = 1; j < 16; ++j) mn = fminf(mn, sc[j]);
+        const float tb = b ? tauf[1] : tauf[0];
+        const uint32_t qsb = b ? qs[1] : qs[0], qeb = b ? qe[1] : qe[0];
+        const float th = inner_b ? tb : (t < qeb && t + kMTile > qsb) ? tb : -__builtin_inff();
+        if (__ballot(mn <= th) == 0) return false;
+        // Rare path: a per-lane bit mask of the scores at or below the lane's
+        // threshold (inside its query's range on a boundary tile); each lane
+        // then stores its set bits to its half of the list, lowest first.  A
+        // score equal to the threshold may be stored needlessly; compaction
+        // drops it.  Keys carry the layout position (bit 31: layout), not the
+        // id; the merge maps positions to ids.
+        uint32_t pm = 0;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) pm |= (sc[j] <= th ? 1u : 0u) << j;
+        if (!inner_b) {
+            const bool full = t >= qsb && t + kMTile <= qeb;
+            if (__ballot(pm != 0 && !full)) pm &= range_bits(t, half, qsb, qeb);
+        }
+        if (b) append_bits(pm, sc, t + 4 * half, lay_bit, mylist[1], cnt[1]);
+        else append_bits(pm, sc, t + 4 * half, lay_bit, mylist[0], cnt[0]);
+        return true;
+    };
     // The loop body, once per tile buffer with a literal buffer number.
 #define FBF_TILE_STEP(U)                                                                              \
     {                                                                                                 \
@@ -787,37 +878,13 @@ _Pragma("unroll")                                                               
             RowFrag<VAR> rf;                                                                          \
             frag_from_lds<VAR>(FBF_BUF(U), lane, rf);                                                 \
             if constexpr (VAR == 1) rf.scale = rscale;                                                \
-_Pragma("unroll")                                                                                     \
-            for (int b = 0; b < kQB; ++b) {                                                           \
-                if (hit[b] == 0) continue;                                                            \
-                f32x16 acc = tile_mma<VAR>(rf, qf[b]);                                                \
-                float sc[16];                                                                         \
-_Pragma("unroll")                                                                                     \
-                for (int j = 0; j < 16; ++j) sc[j] = (VAR == 1 || VAR == 3) ? acc[j] + xn[j] : acc[j]; \
-/* One compare per lane: the smallest of its 16 scores against the    */                           \
-/* lane's threshold, or against -inf when the tile lies wholly outside */                            \
-/* the lane's query range (the block runs for the other queries).      */                            \
-                float mn = sc[0];                                                                     \
-_Pragma("unroll")                                                                                     \
-                for (int j = 1; j < 16; ++j) mn = fminf(mn, sc[j]);                                   \
-                const float th = inner[b] ? tauf[b]                                                   \
-                                 : (t < qe[b] && t + kMTile > qs[b]) ? tauf[b] : -__builtin_inff();   \
-                if (__ballot(mn <= th) == 0) continue;                                                \
-/* Rare path: a per-lane bit mask of the scores at or below the lane's  */                           \
-/* threshold (inside its query's range on a boundary tile); each lane    */                           \
-/* then stores its set bits to its half of the list, lowest first.  A    */                           \
-/* score equal to the threshold may be stored needlessly; compaction     */                           \
-/* drops it.  Keys carry the layout position (bit 31: layout), not the   */                           \
-/* id; the merge maps positions to ids.                                  */                           \
-                appended = true;                                                                      \
-                uint32_t pm = 0;                                                                      \
-_Pragma("unroll")                                                                                     \
-                for (int j = 0; j < 16; ++j) pm |= (sc[j] <= th ? 1u : 0u) << j;                      \
-                if (!inner[b]) {                                                                      \
-                    const bool full = t >= qs[b] && t + kMTile <= qe[b];                              \
-                    if (__ballot(pm != 0 && !full)) pm &= range_bits(t, half, qs[b], qe[b]);          \
-                }                                                                                     \
-                append_bits(pm, sc, t + 4 * half, lay_bit, mylist[b], cnt[b]);                        \
+            if (hit[0] != 0) {                                                                        \
+                const f32x16 acc = tile_mma<VAR>(rf, qf[0]);                                          \
+                appended |= test_block(acc, xn, t, 0, inner[0]);                                      \
+            }                                                                                         \
+            if (hit[1] != 0) {                                                                        \
+                const f32x16 acc = tile_mma<VAR>(rf, qf[1]);                                          \
+                appended |= test_block(acc, xn, t, 1, inner[1]);                                      \
             }                                                                                         \
         }                                                                                             \
 /* Wave-local list maintenance (no cross-wave synchronisation): halves */                             \
@@ -839,7 +906,7 @@ _Pragma("unroll 1")                                                             
                     uint32_t c0 = __builtin_amdgcn_readlane(cb, c);                                   \
                     uint32_t c1 = __builtin_amdgcn_readlane(cb, 32 + c);                              \
                     uint64_t tau = tau_w[jj];                                                         \
-                    compact_split<B>(list_of(jj), c0, c1, tau, a.K, sbuf);                            \
+                    compact_split<B>(list_base(b, c), c0, c1, tau, a.K, sbuf);                        \
                     if (col == c) {                                                                   \
                         if (b) cnt[1] = half ? c1 : c0;                                               \
                         else cnt[0] = half ? c1 : c0;                                                 \
@@ -878,28 +945,31 @@ _Pragma("unroll")                                                               
     }
 #undef FBF_TILE_STEP
 #undef FBF_BUF
-    // Final cut: each list becomes contiguous, [0, count): longer than K ->
-    // sorted and cut to K; otherwise its second half moves up behind the
-    // first (unsorted: the merge does not need order).
-    WaveSync::sync();
-    for (uint32_t j = 0; j < nq; ++j) {
-        const uint32_t b = j >> 5, c = j & 31;
+    // Final cut: lists longer than K are sorted and cut to K (halves of
+    // ceil(K / 2) and the rest); every list's count word names its two
+    // halves, [0, c0) and [H, H + c1), which the merge reads as one list.
+#pragma unroll
+    for (int b = 0; b < kQB; ++b) {
         const uint32_t cb = b ? cnt[1] : cnt[0];
-        uint32_t c0 = __builtin_amdgcn_readlane(cb, c), c1 = __builtin_amdgcn_readlane(cb, 32 + c);
-        uint64_t *list = list_of(j);
-        uint32_t count = c0 + c1;
-        if (count > a.K) {
-            uint64_t tau = tau_w[j];
-            compact_split<B>(list, c0, c1, tau, a.K, sbuf);
-            if (lane == 0) tau_w[j] = tau;
-            count = c0 + c1;
+        const uint32_t other = (uint32_t)__shfl_xor((int)cb, 32);
+        const uint64_t over = __ballot(half == 0 && qv[b] && cb + other > a.K);
+        uint32_t need = (uint32_t)over;
+        while (need) {
+            const uint32_t c = (uint32_t)__builtin_ctz(need);
+            need &= need - 1;
+            const uint32_t jj = 32 * b + c;
+            uint32_t c0 = __builtin_amdgcn_readlane(cb, c), c1 = __builtin_amdgcn_readlane(cb, 32 + c);
+            uint64_t tau = tau_w[jj];
+            compact_split<B>(list_base(b, c), c0, c1, tau, a.K, sbuf);
+            if (col == c) {
+                if (b) cnt[1] = half ? c1 : c0;
+                else cnt[0] = half ? c1 : c0;
+            }
+            if (lane == 0) tau_w[jj] = tau;
         }
-        if (c1) {
-            // [H, H + c1) -> [c0, c0 + c1); c0 + c1 <= K <= H, so the ranges are disjoint.
-            WaveSync::sync();
-            for (uint32_t i = lane; i < c1; i += 64) list[c0 + i] = list[H + i];
-        }
-        if (lane == 0 && (a.qmask == nullptr || a.qmask[qi_of(j)] != 0)) a.counts[slot_of(j)] = count;
+        const uint32_t cf = b ? cnt[1] : cnt[0];
+        const uint32_t of = (uint32_t)__shfl_xor((int)cf, 32);
+        if (half == 0 && qv[b]) a.counts[myslot[b]] = list_count_word(cf, of, H);
     }
     share_thresholds<WaveSync>(a.gtau, nq, tau_w, qi_of);
 }
