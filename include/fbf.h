@@ -51,7 +51,7 @@ typedef enum fbf_mode {
     FBF_MODE_SQ8SYM = 4,    /* symmetric int8 scan (router's bruteforce()), FP32 rerank of k' */
     FBF_MODE_EXACT_BF16 = 5 /* the FBF_MODE_EXACT_F32 results (same ids and distances, Bruteforce
                                utils.h:80-158) from the BF16 MFMA scan and an FP32 rerank of k'
-                               (default 720; rows per work item default 262144) candidates: a
+                               (default 720; work items of 256 queries over 65536 rows) candidates: a
                                query is certified when the k'-th BF16
                                score, less a bound on its rounding error, exceeds the k-th exact
                                distance; the others are searched again by the FP32 scan.  Needs
@@ -72,12 +72,13 @@ typedef struct fbf_dataset fbf_dataset;
 
 /* Search tuning.  Zero fields take the library defaults.  kprime is the
  * over-fetch depth of the approximate modes (ignored by FBF_MODE_EXACT_F32);
- * chunk_rows is the rows per work item (default 262144); group_queries the
+ * chunk_rows is the rows per work item (default: from the batch's total span, 16 Ki to 256 Ki
+ * rows, about 32 Ki work items per batch); group_queries the
  * queries sharing one streamed tile (<= 16 for the VALU scans, default 16;
  * <= 256 for the MFMA scans, default 128). */
 typedef struct fbf_search_opts {
     uint32_t mode;        /* fbf_mode */
-    uint32_t kprime;      /* 0 -> 3k/2 (BF16, SQ8), 4k (FP8, SQ8SYM) or 720 (EXACT_BF16), at most 896 */
+    uint32_t kprime;      /* 0 -> 3k/2 (BF16, SQ8), 4k (SQ8SYM), 9k (FP8) or 720 (EXACT_BF16), at most 896 */
     uint32_t chunk_rows;  /* 0 -> default */
     uint32_t group_queries;
     uint32_t persistent;  /* scan launch form: 0 -> default (MFMA modes: persistent; others: one

@@ -63,7 +63,7 @@ def recall_at_k(ids: np.ndarray, ref: np.ndarray) -> float:
 
 def tune(ds, queries, metas, k: int = 100, mode: str = "bf16", recall_target: float = 0.995,
          sample: int = 256, groups=(128, 256), chunks=(32768, 65536, 131072),
-         kprime_factors=(1.0, 1.1, 1.2, 1.5, 2.0, 4.0), repeats: int = 1, tie: float = 0.03,
+         kprime_factors=(1.0, 1.1, 1.2, 1.5, 2.0, 4.0, 6.0, 9.0), repeats: int = 1, tie: float = 0.03,
          log=None) -> TuneResult:
     """queries: torch device float32 [nq, d]; metas: numpy float32 [nq, 4]."""
     import torch

@@ -835,10 +835,14 @@ static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, 
     uint32_t K = k;
     if (!exact) {
         // Default over-fetch: 3k/2 for the BF16 and SQ8 scans (their score
-        // errors are small against the gaps between neighbours), 4k for FP8
-        // and the symmetric int8 codes.
+        // errors are small against the gaps between neighbours), 4k for the
+        // symmetric int8 codes, 9k (at most 896) for FP8: e4m3 keeps three
+        // mantissa bits, and on the 10 M-row Gaussian-mixture workload its
+        // score errors reach the gap between the 100th and the 400th
+        // neighbour (recall@100 0.957 at k' = 400, 0.994 at 896).
         const bool fine = o.mode == FBF_MODE_BF16 || o.mode == FBF_MODE_SQ8;
-        K = o.kprime ? o.kprime : std::min<uint32_t>(fine ? (3 * k + 1) / 2 : 4 * k, 896);
+        K = o.kprime ? o.kprime
+                     : std::min<uint32_t>(fine ? (3 * k + 1) / 2 : o.mode == FBF_MODE_FP8 ? 9 * k : 4 * k, 896);
         if (K < k) return fail(FBF_ERR_INVALID_ARG, "kprime must be >= k");
         if (K > 896) return fail(FBF_ERR_INVALID_ARG, "kprime must be <= 896");
     }
@@ -1198,9 +1202,14 @@ static fbf_status search_exact_bf16(fbf_dataset *ds, const float *queries_dev, c
     // faster than the general defaults (the main pass appends fewer keys).
     if (ob.prepass_div == 0) ob.prepass_div = 8;
     if (ob.prepass_margin == 0) ob.prepass_margin = 200;
-    // Lists of up to 1024 keys: 256 Ki rows per work item by default keeps
-    // their memory (slots x 8 KiB) near 12 GB for 100 k queries over 10 M rows.
-    if (ob.chunk_rows == 0) ob.chunk_rows = 262144;
+    // Items of 256 queries over 64 Ki rows: the groups of one chunk share
+    // its rows in L2 (C1: 243 ms per 100 k queries, against 258 ms with
+    // 256 Ki-row items of the default group size and 277 ms with 128 Ki);
+    // the lists of up to 1024 keys then take slots x 8 KiB, about 40 GB for
+    // 100 k queries over 10 M rows.  Sub-batches (max_batch) bound that
+    // memory at some speed (50 k queries: 333 ms).
+    if (ob.chunk_rows == 0) ob.chunk_rows = 65536;
+    if (ob.group_queries == 0) ob.group_queries = 256;
     Scratch &S = ds->scr;
     HIP_TRY(S.d_cert.reserve(nq * sizeof(uint32_t)));
     s = search_impl(ds, S, true, queries_dev, metas, nq, k, &ob, out_ids_dev, out_dists_dev, st, nullptr,
@@ -1332,11 +1341,17 @@ extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, cons
         fbf_status s = device_ranges(ds, metas, nq, (hipStream_t)stream, qr);
         if (s != FBF_OK) return s;
         const uint64_t max_batch = opts->max_batch ? opts->max_batch : 131072u;
+        uint64_t fallbacks = 0;  // summed over the sub-batches
         for (uint64_t i0 = 0; i0 < nq; i0 += max_batch) {
             const uint64_t m = std::min<uint64_t>(max_batch, nq - i0);
             if (opts->mode == FBF_MODE_EXACT_BF16) {
                 s = search_exact_bf16(ds, queries_dev + i0 * ds->d, nullptr, m, k, *opts, out_ids_dev + i0 * k,
                                       out_dists_dev + i0 * k, (hipStream_t)stream, qr.data() + i0);
+                if (s == FBF_OK) {
+                    std::lock_guard<std::mutex> lock(ds->mu);
+                    fallbacks += ds->last_fallbacks;
+                    ds->last_fallbacks = fallbacks;
+                }
             } else {
                 {
                     std::lock_guard<std::mutex> lock(ds->mu);
@@ -1353,11 +1368,15 @@ extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, cons
     const uint64_t max_batch = opts && opts->max_batch ? opts->max_batch : 131072u;
     if (nq > max_batch) {
         if (!metas || !queries_dev || !out_ids_dev || !out_dists_dev) return fail(FBF_ERR_INVALID_ARG, "null argument");
+        uint64_t fallbacks = 0;  // summed over the sub-batches
         for (uint64_t i0 = 0; i0 < nq; i0 += max_batch) {
             const uint64_t m = std::min<uint64_t>(max_batch, nq - i0);
             const fbf_status s = fbf_search(ds, queries_dev + i0 * ds->d, metas + 4 * i0, m, k, opts,
                                             out_ids_dev + i0 * k, out_dists_dev + i0 * k, stream);
             if (s != FBF_OK) return s;
+            std::lock_guard<std::mutex> lock(ds->mu);
+            fallbacks += ds->last_fallbacks;
+            ds->last_fallbacks = fallbacks;
         }
         return FBF_OK;
     }

@@ -106,7 +106,7 @@ def test_c1_exact_bf16_equals_exact(c1, exact):
     assert np.array_equal(bd, d)
 
 
-MODES = [("bf16", 0), ("bf16", 400), ("sq8", 0), ("sq8", 400), ("sq8sym", 0), ("fp8", 400)]
+MODES = [("bf16", 0), ("bf16", 400), ("sq8", 0), ("sq8", 400), ("sq8sym", 0), ("fp8", 0), ("fp8", 400)]
 
 
 @pytest.mark.parametrize("mode,kprime", MODES)
@@ -114,7 +114,7 @@ def test_c1_mode_invariants_and_recall(orc, c1, exact, mode, kprime):
     ids_e, d_e = exact
     _say(f"case: N=10M nq=100k mode={mode} k'={kprime or 'default'}")
     ids, d = _search(c1, mode=mode, kprime=kprime)
-    kp = kprime or {"bf16": 150, "sq8": 150, "sq8sym": 400, "fp8": 400}[mode]
+    kp = kprime or {"bf16": 150, "sq8": 150, "sq8sym": 400, "fp8": 896}[mode]
     covered = sc.check_invariants(orc, c1["vecs"], c1["lab"], c1["ts"], c1["q"], c1["metas"], c1["counts"],
                                   ids, d, K, kprime=kp, exact_ids=ids_e, exact_d=d_e)
     rec = sc.recall_vs(orc, ids, ids_e)
@@ -122,8 +122,10 @@ def test_c1_mode_invariants_and_recall(orc, c1, exact, mode, kprime):
     _say(f"  recall@100 vs exact over 100k: {rec:.5f}; by type "
          + ", ".join(f"{t}: {r:.4f}" for t, r in by_type.items()) + f"; {covered} queries exact by k'")
     floor = 0.99
-    if mode == "fp8":
-        # e4m3 holds 3 mantissa bits: measured below target at k' = 400.
+    if mode == "fp8" and kprime == 400:
+        # e4m3 keeps three mantissa bits: at the configs' k' = 400 its score
+        # errors reach the gap between the 100th and 400th neighbour (0.957);
+        # the mode's default depth (k' = 896) meets the target.
         floor = 0.95
     assert rec >= floor
 

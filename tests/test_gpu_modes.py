@@ -19,7 +19,12 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 MODES = ["bf16", "sq8", "fp8", "sq8sym"]
-RECALL_FLOOR = {"bf16": 0.99, "sq8": 0.99, "fp8": 0.95, "sq8sym": 0.99}
+RECALL_FLOOR = {"bf16": 0.99, "sq8": 0.99, "fp8": 0.99, "sq8sym": 0.99}
+# Over-fetch depth per mode: the configs' k' = 400, and FP8's default (9k, at
+# most 896): e4m3 needs that depth for recall@100 >= 0.99.
+KPRIME = {"bf16": 400, "sq8": 400, "fp8": 896, "sq8sym": 400}
+# k = 10 at k' = 4k over 64 queries.
+SMALL_K_FLOOR = {"bf16": 0.94, "sq8": 0.94, "fp8": 0.90, "sq8sym": 0.94}
 
 
 def _say(msg):
@@ -51,10 +56,11 @@ def _search(ds, q, metas, k, mode, **kw):
 
 @pytest.mark.parametrize("mode", MODES)
 def test_mode_invariants_and_recall(pkg, orc, c0, mode):
-    _say(f"case: config0 N=10000 nq=400 mode={mode} k=100 k'=400")
+    kp = KPRIME[mode]
+    _say(f"case: config0 N=10000 nq=400 mode={mode} k=100 k'={kp}")
     vecs, labels, ts, q, metas, ds = c0
     lab = labels.astype(np.int32)
-    ids, d = _search(ds, q, metas, 100, mode, kprime=400)
+    ids, d = _search(ds, q, metas, 100, mode, kprime=kp)
     eid, ed = orc.exact_predicate(vecs, lab, ts, q, metas, k=100, mode="canon")
     L = orc.Layouts(lab, ts)
     for i in range(q.shape[0]):
@@ -69,7 +75,7 @@ def test_mode_invariants_and_recall(pkg, orc, c0, mode):
             assert d[i, j] == np.float32(orc.l2_canon(vecs[r], q[i]))
         keys = orc.make_keys(d[i][valid], ids[i][valid])
         assert np.all(np.diff(keys.astype(np.float64)) > 0) or valid.sum() <= 1
-        if cnt <= 400:
+        if cnt <= kp:
             assert np.array_equal(ids[i], eid[i]), (mode, i, cnt)
             assert np.array_equal(d[i], ed[i].astype(np.float32))
     rec, _ = orc.recall(ids, eid)
@@ -108,7 +114,7 @@ def test_mode_small_kprime_and_chunks(pkg, orc, c0, mode):
     assert np.array_equal(ids, ids2) and np.array_equal(d, d2)
     eid, ed = orc.exact_predicate(vecs, labels.astype(np.int32), ts, q[:64], metas[:64], k=10, mode="canon")
     rec, _ = orc.recall(ids, eid)
-    assert rec >= RECALL_FLOOR[mode] - 0.05
+    assert rec >= SMALL_K_FLOOR[mode]
     for i in range(64):
         valid = ids[i] != 0xFFFFFFFF
         for j in np.nonzero(valid)[0]:
