@@ -748,8 +748,15 @@ static MergeArgs merge_args(const Scratch &S, int B, uint64_t nq, uint32_t k, ui
 }
 
 // ---------------------------------------------------------------- search
-// Ranges whose prefix would be shorter than this are scanned in one pass.
-constexpr uint64_t kMinPrefixRows = 16384;
+// Ranges whose prefix would be shorter than this (or than twice the keys it
+// keeps) are scanned in one pass.  An unseeded range starts with an open
+// threshold and compacts its lists many times while they fill -- at large
+// k' (720, 896: lists of 1024 keys with room for about 128 appends between
+// compactions) that dominated the scan.  C1, ids identical: the certified
+// exact mode 243 -> 167 ms per 100 k queries, FP8 at k' = 896 336 -> 226 ms,
+// BF16 at k' = 110 90.9 -> 88.1 ms, against a minimum of 16 Ki rows and
+// eight times the kept keys.
+constexpr uint64_t kMinPrefixRows = 256;
 // Work items in descending cost order (largest first), grouped by layout
 // when each layout is its own launch: big items start early and the small ones
 // fill the tail.
@@ -905,17 +912,18 @@ static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, 
     // thousands an unseeded scan appends while its lists fill.  The merge
     // verifies each seeded query (exact when its k'-th merged key is <= tau0)
     // and the few that fail are rescanned over their whole range.
-    // Defaults 1/10 and 250%: on C1 every MFMA mode runs 0.5-2 ms faster than
-    // with 1/8 and 200%, and no seed fails.
-    const uint32_t pdiv = mfma ? (o.prepass_div ? o.prepass_div : 10u) : 1u;
-    const uint32_t margin = o.prepass_margin ? o.prepass_margin : 250u;
+    // Defaults 1/16 and 300%: on C1 (BF16, k' = 110) 88.3 ms per step against
+    // 89.9 with 1/10 and 250%, and no seed fails; 1/24 or 1/32 with 250-300%
+    // let seeds fail (rescans), and 400% appends more than it saves.
+    const uint32_t pdiv = mfma ? (o.prepass_div ? o.prepass_div : 16u) : 1u;
+    const uint32_t margin = o.prepass_margin ? o.prepass_margin : 300u;
     const uint32_t kpre = (uint32_t)std::min<uint64_t>(K, std::max<uint64_t>(1, ((uint64_t)K * margin + 100ull * pdiv - 1) / (100ull * pdiv)));
     Plan P;
     uint64_t n_items_a = 0;
     bool seeded = false;
     if (pdiv > 1) {
         std::vector<QRange> qa = qr, qb = qr;  // prefixes (empty when not split), the rest
-        const uint64_t min_prefix = std::max<uint64_t>(kMinPrefixRows, 8ull * kpre);
+        const uint64_t min_prefix = std::max<uint64_t>(kMinPrefixRows, 2ull * kpre);
         uint64_t max_prefix = 0;
         for (uint64_t i = 0; i < nq; ++i) {
             const uint64_t len = qr[i].e - qr[i].s;
@@ -1198,9 +1206,10 @@ static fbf_status search_exact_bf16(fbf_dataset *ds, const float *queries_dev, c
     // are certified; a larger k' leaves the 1024-key lists too little room
     // above k' / 2 between compactions, a smaller one certifies fewer.
     if (ob.kprime == 0) ob.kprime = 720;
-    // At this k' a prefix over 1/8 of each range with a 200% margin is
-    // faster than the general defaults (the main pass appends fewer keys).
-    if (ob.prepass_div == 0) ob.prepass_div = 8;
+    // At this k' a prefix over 1/16 of each range with a 200% margin is
+    // faster than the general defaults (C1: 143 ms per step against 162 with
+    // 300%, 148 with 1/24 and 250%).
+    if (ob.prepass_div == 0) ob.prepass_div = 16;
     if (ob.prepass_margin == 0) ob.prepass_margin = 200;
     // Items of 256 queries over 64 Ki rows: the groups of one chunk share
     // its rows in L2 (C1: 243 ms per 100 k queries, against 258 ms with
