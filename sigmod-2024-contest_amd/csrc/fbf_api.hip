@@ -863,7 +863,10 @@ static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, 
     // prefix lists start empty, so its waves compact often, and every
     // compaction holds up the other waves that share the tile ring.
     const uint32_t gmax_pre = std::min<uint32_t>(gmax, 128u);
-    auto waves_of = [](uint32_t g) { return g <= 64 ? 1 : g <= 128 ? 2 : 4; };
+    // Waves per MFMA work item: each wave holds kMfmaWaveQueries queries.
+    auto waves_of = [&](uint32_t g) {
+        return !mfma ? 1 : g <= kMfmaWaveQueries ? 1 : g <= 2 * kMfmaWaveQueries ? 2 : 4;
+    };
     if (o.persistent > 2) return fail(FBF_ERR_INVALID_ARG, "persistent must be 0, 1 or 2");
     const bool persist = o.persistent == 1 || (o.persistent == 0 && mfma);
     // Item order: chunk-major runs per XCD for the persistent MFMA scans, else

@@ -49,6 +49,8 @@ __device__ __host__ inline uint32_t list_count_c1(uint32_t w) { return (w >> 11)
 __device__ __host__ inline uint32_t list_count_h(uint32_t w) { return (w >> 22) << 6; }
 
 constexpr int kMaxGroup = 16;    // queries per work item
+// Queries one wave of the MFMA scans holds: 32-query blocks, two or four.
+constexpr uint32_t kMfmaWaveQueries = 64;
 
 // (distance, id) key: distance bits in the high word.  For the non-negative
 // distances of the exact scan the float bits are already ordered; the

@@ -617,7 +617,9 @@ __device__ inline float tau_as_float(uint64_t tau) {
 // cut to the K smallest keys, split evenly between the halves; the threshold
 // becomes the K-th key.  At the end of the item each list is made contiguous
 // (its second half moved up, or the whole list cut to K).
-constexpr int kQB = 2;  // 32-query blocks per wave
+constexpr int kQB = (int)(kMfmaWaveQueries / 32);  // 32-query blocks per wave (2 or 4)
+constexpr uint32_t kWaveQ = kMfmaWaveQueries;          // queries per wave
+static_assert(kQB == 2 || kQB == 4, "two or four query blocks per wave");
 constexpr uint32_t kShareTiles = 512;  // tiles between threshold exchanges without a compaction
 
 // Bit j (score j of the lane: row r0 + 8 (j / 4) + (j % 4), r0 = tile row +
@@ -702,15 +704,16 @@ __device__ __attribute__((always_inline)) void scan_mfma_item(const ScanMfmaArgs
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t col = lane & 31, half = lane >> 5;
     const uint32_t nq_all = W.q_count;
-    const uint32_t q0 = 64 * wave;                            // this wave's first query of the item
-    const uint32_t nq = nq_all > q0 ? (nq_all - q0 < 64 ? nq_all - q0 : 64) : 0;
+    const uint32_t q0 = kWaveQ * wave;                        // this wave's first query of the item
+    const uint32_t nq = nq_all > q0 ? (nq_all - q0 < kWaveQ ? nq_all - q0 : kWaveQ) : 0;
     uint64_t *tau_w = s_tau + q0;
     const LayoutMfma &L = a.lay[W.layout];
     const uint4 *gA = reinterpret_cast<const uint4 *>(L.tiles);
 
     // Per lane and block: the column's query (j = 32 b + col of the wave's
     // queries), its range, its list slot and this lane's half of the list.
-    // Lane l holds query l in block l / 32 (col = l % 32, half = l / 32).
+    // Lane l holds query l + 64 r in block 2 r + l / 32 (col = l % 32,
+    // half = l / 32).
     QFrag<VAR> qf[kQB];
     float tauf[kQB];
     uint32_t qs[kQB], qe[kQB], qib[kQB];
@@ -736,9 +739,11 @@ __device__ __attribute__((always_inline)) void scan_mfma_item(const ScanMfmaArgs
         cnt[b] = 0;
         mylist[b] = a.lists + myslot[b] * (uint64_t)a.list_stride + half * H;
     }
-    const uint32_t my_qi = half ? qib[1] : qib[0];  // query j = lane
-    // Every per-query loop below runs j = lane only (nq <= 64).
-    auto qi_of = [&](uint32_t) { return my_qi; };
+    // The queries j = lane (and j = lane + 64 with four blocks): every
+    // per-query loop below runs over exactly those j.
+    const uint32_t my_qi0 = half ? qib[1] : qib[0];
+    const uint32_t my_qi1 = kQB == 4 ? (half ? qib[kQB - 1] : qib[kQB - 2]) : my_qi0;
+    auto qi_of = [&](uint32_t j) { return j >= 64 ? my_qi1 : my_qi0; };
     // Base of the list of query j = 32 b + c (held by lane c, half 0).
     auto list_base = [&](int b, uint32_t c) {
         const uint64_t p = reinterpret_cast<uint64_t>(mylist[b]);
@@ -747,8 +752,8 @@ __device__ __attribute__((always_inline)) void scan_mfma_item(const ScanMfmaArgs
     };
 
     // Start from the thresholds earlier items (or the prefix pass) published.
-    if (lane < nq)
-        tau_w[lane] = a.gtau ? __hip_atomic_load(&a.gtau[my_qi], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : kKeyMax;
+    for (uint32_t j = lane; j < nq; j += 64)
+        tau_w[j] = a.gtau ? __hip_atomic_load(&a.gtau[qi_of(j)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : kKeyMax;
 #pragma unroll
     for (int b = 0; b < kQB; ++b) load_query<VAR>(a, qv[b], qib[b], lane, qf[b]);
     WaveSync::sync();
@@ -809,8 +814,8 @@ __device__ __attribute__((always_inline)) void scan_mfma_item(const ScanMfmaArgs
         float mn = sc[0];
 #pragma unroll
         for (int j = 1; j < 16; ++j) mn = fminf(mn, sc[j]);
-        const float tb = b ? tauf[1] : tauf[0];
-        const uint32_t qsb = b ? qs[1] : qs[0], qeb = b ? qe[1] : qe[0];
+        const float tb = tauf[b];
+        const uint32_t qsb = qs[b], qeb = qe[b];
         const float th = inner_b ? tb : (t < qeb && t + kMTile > qsb) ? tb : -__builtin_inff();
         if (__ballot(mn <= th) == 0) return false;
         // Rare path: a per-lane bit mask of the scores at or below the lane's
@@ -826,8 +831,7 @@ __device__ __attribute__((always_inline)) void scan_mfma_item(const ScanMfmaArgs
             const bool full = t >= qsb && t + kMTile <= qeb;
             if (__ballot(pm != 0 && !full)) pm &= range_bits(t, half, qsb, qeb);
         }
-        if (b) append_bits(pm, sc, t + 4 * half, lay_bit, mylist[1], cnt[1]);
-        else append_bits(pm, sc, t + 4 * half, lay_bit, mylist[0], cnt[0]);
+        append_bits(pm, sc, t + 4 * half, lay_bit, mylist[b], cnt[b]);
         return true;
     };
     // The loop body, once per tile buffer with a literal buffer number.
@@ -878,13 +882,11 @@ _Pragma("unroll")                                                               
             RowFrag<VAR> rf;                                                                          \
             frag_from_lds<VAR>(FBF_BUF(U), lane, rf);                                                 \
             if constexpr (VAR == 1) rf.scale = rscale;                                                \
-            if (hit[0] != 0) {                                                                        \
-                const f32x16 acc = tile_mma<VAR>(rf, qf[0]);                                          \
-                appended |= test_block(acc, xn, t, 0, inner[0]);                                      \
-            }                                                                                         \
-            if (hit[1] != 0) {                                                                        \
-                const f32x16 acc = tile_mma<VAR>(rf, qf[1]);                                          \
-                appended |= test_block(acc, xn, t, 1, inner[1]);                                      \
+_Pragma("unroll")                                                                                     \
+            for (int b = 0; b < kQB; ++b) {                                                           \
+                if (hit[b] == 0) continue;                                                            \
+                const f32x16 acc = tile_mma<VAR>(rf, qf[b]);                                          \
+                appended |= test_block(acc, xn, t, b, inner[b]);                                      \
             }                                                                                         \
         }                                                                                             \
 /* Wave-local list maintenance (no cross-wave synchronisation): halves */                             \
@@ -894,9 +896,9 @@ _Pragma("unroll")                                                               
         since_share += live ? 1u : 0u;                                                                \
         if (appended || since_share >= kShareTiles) {                                                 \
             bool changed = false;                                                                     \
-_Pragma("unroll 1")                                                                                   \
+_Pragma("unroll")                                                                                     \
             for (int b = 0; b < kQB; ++b) {                                                           \
-                const uint32_t cb = b ? cnt[1] : cnt[0];                                              \
+                const uint32_t cb = cnt[b];                                                           \
                 const uint64_t nb = __ballot(cb > H - 16);                                            \
                 uint32_t need = (uint32_t)nb | (uint32_t)(nb >> 32);                                  \
                 while (need) {                                                                        \
@@ -907,10 +909,7 @@ _Pragma("unroll 1")                                                             
                     uint32_t c1 = __builtin_amdgcn_readlane(cb, 32 + c);                              \
                     uint64_t tau = tau_w[jj];                                                         \
                     compact_split<B>(list_base(b, c), c0, c1, tau, a.K, sbuf);                        \
-                    if (col == c) {                                                                   \
-                        if (b) cnt[1] = half ? c1 : c0;                                               \
-                        else cnt[0] = half ? c1 : c0;                                                 \
-                    }                                                                                 \
+                    if (col == c) cnt[b] = half ? c1 : c0;                                            \
                     if (lane == 0) tau_w[jj] = tau;                                                   \
                     changed = true;                                                                   \
                 }                                                                                     \
@@ -950,7 +949,7 @@ _Pragma("unroll")                                                               
     // halves, [0, c0) and [H, H + c1), which the merge reads as one list.
 #pragma unroll
     for (int b = 0; b < kQB; ++b) {
-        const uint32_t cb = b ? cnt[1] : cnt[0];
+        const uint32_t cb = cnt[b];
         const uint32_t other = (uint32_t)__shfl_xor((int)cb, 32);
         const uint64_t over = __ballot(half == 0 && qv[b] && cb + other > a.K);
         uint32_t need = (uint32_t)over;
@@ -961,13 +960,10 @@ _Pragma("unroll")                                                               
             uint32_t c0 = __builtin_amdgcn_readlane(cb, c), c1 = __builtin_amdgcn_readlane(cb, 32 + c);
             uint64_t tau = tau_w[jj];
             compact_split<B>(list_base(b, c), c0, c1, tau, a.K, sbuf);
-            if (col == c) {
-                if (b) cnt[1] = half ? c1 : c0;
-                else cnt[0] = half ? c1 : c0;
-            }
+            if (col == c) cnt[b] = half ? c1 : c0;
             if (lane == 0) tau_w[jj] = tau;
         }
-        const uint32_t cf = b ? cnt[1] : cnt[0];
+        const uint32_t cf = cnt[b];
         const uint32_t of = (uint32_t)__shfl_xor((int)cf, 32);
         if (half == 0 && qv[b]) a.counts[myslot[b]] = list_count_word(cf, of, H);
     }
@@ -996,7 +992,7 @@ void scan_mfma_kernel(ScanMfmaArgs a) {
     __shared__ uint4 sT0[64 * TileChunks<VAR>::value], sT1[64 * TileChunks<VAR>::value],
         sT2[64 * TileChunks<VAR>::value], sT3[64 * TileChunks<VAR>::value];
     __shared__ uint64_t sbuf[NW][B];
-    __shared__ uint64_t s_tau[64 * NW];
+    __shared__ uint64_t s_tau[kWaveQ * NW];
     __shared__ uint32_t s_idx;
     uint64_t *my_sbuf = sbuf[threadIdx.x >> 6];
     // One workgroup per item (queue == null: item blockIdx.x), or the
