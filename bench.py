@@ -424,11 +424,121 @@ def dropin(a, ge, fbf, cfg, vecs, labels, ts, q, metas) -> int:
     res["gpu_batch"] = routed.size / (time.perf_counter() - t)
     same = sum(set(got[j].tolist()) == set(ref_out[j].tolist()) for j in range(ncpu))
     fbf.unregister_codes(cb); fbf.unregister_codes(ct)
+    # The same calls from a C++ program (16 OpenMP threads, no Python in the
+    # loop): the drop-in header's bruteforce() on the GPU, then the
+    # reference's own bruteforce() from oracle/_ref on this host's CPU.
+    cpp = dropin_cpp(ge, cb, ct, rr[routed], qc, alpha, R is not None)
+    if cpp:
+        res.update(cpp)
     line = {"dropin": cfg.name, "routed_queries": int(routed.size), "calls_per_s": res,
             "cpu_kind": "reference" if R is not None else "port", "id_sets_equal": f"{same}/{ncpu}",
             "mean_slice_rows": float(cnt[routed].mean()), **host_cpu()}
     print(json.dumps(line), flush=True)
     return 0
+
+
+DROPIN_CPP = r"""
+#define FBF_REPLACE_REFERENCE_BRUTEFORCE
+#include "fbf_bruteforce.hpp"
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <dlfcn.h>
+#include <omp.h>
+#include <vector>
+
+// argv: cb.bin ct.bin q.bin slices.bin (n x 3 u64: layout, s, e) alpha ref_so|- threads
+static bool slurp(const char *p, std::vector<char> &v) {
+    FILE *f = std::fopen(p, "rb"); if (!f) return false;
+    std::fseek(f, 0, SEEK_END); long n = std::ftell(f); std::fseek(f, 0, SEEK_SET);
+    v.resize((size_t)n); bool ok = std::fread(v.data(), 1, (size_t)n, f) == (size_t)n; std::fclose(f); return ok;
+}
+typedef void (*ref_bf_t)(uint64_t, int32_t *, uint64_t, const int8_t *, float, const int8_t *);
+int main(int argc, char **argv) {
+    if (argc != 8) return 2;
+    std::vector<char> cb, ct, q, sl;
+    if (!slurp(argv[1], cb) || !slurp(argv[2], ct) || !slurp(argv[3], q) || !slurp(argv[4], sl)) return 3;
+    const float alpha = std::strtof(argv[5], nullptr);
+    const int threads = std::atoi(argv[7]);
+    const uint64_t *se = reinterpret_cast<const uint64_t *>(sl.data());
+    const size_t nq = q.size() / 112;
+    fbf_ref::check_or_abort(fbf_register_codes(cb.data(), cb.size()), "fbf_register_codes");
+    fbf_ref::check_or_abort(fbf_register_codes(ct.data(), ct.size()), "fbf_register_codes");
+    std::vector<int> gpu(nq * 140, -1), cpu(nq * 140, -1);
+    omp_set_num_threads(threads);
+    auto run = [&](std::vector<int> &out, ref_bf_t fn, size_t count) {
+#pragma omp parallel for schedule(dynamic)
+        for (size_t i = 0; i < count; ++i) {
+            char *arr = se[3 * i] == 0 ? cb.data() : ct.data();
+            const size_t s = se[3 * i + 1], e = se[3 * i + 2], K = std::min<size_t>(140, e - s);
+            int8_t *qi = reinterpret_cast<int8_t *>(q.data() + i * 112);
+            if (fn) fn(K, out.data() + i * 140, e - s, reinterpret_cast<const int8_t *>(arr + s * 112), alpha, qi);
+            else bruteforce(100, K, out.data() + i * 140, e - s, arr + s * 112, alpha, 1, qi);
+        }
+    };
+    run(gpu, nullptr, std::min<size_t>(nq, 64));  // warm-up: streams, engines
+    auto t0 = std::chrono::steady_clock::now();
+    run(gpu, nullptr, nq);
+    double tg = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    double tc = -1.0;
+    size_t same = 0;
+    if (argv[6][0] != '-') {
+        void *h = dlopen(argv[6], RTLD_NOW);
+        ref_bf_t fn = h ? (ref_bf_t)dlsym(h, "ref_bruteforce") : nullptr;
+        if (fn) {
+            t0 = std::chrono::steady_clock::now();
+            run(cpu, fn, nq);
+            tc = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            for (size_t i = 0; i < nq; ++i) {
+                std::vector<int> a(gpu.begin() + i * 140, gpu.begin() + i * 140 + 140);
+                std::vector<int> b(cpu.begin() + i * 140, cpu.begin() + i * 140 + 140);
+                std::sort(a.begin(), a.end()); std::sort(b.begin(), b.end());
+                same += a == b;
+            }
+        }
+    }
+    std::printf("{\"cpp_gpu_per_call\": %.1f, \"cpp_cpu_per_call\": %.1f, \"cpp_id_sets_equal\": %zu, \"cpp_calls\": %zu, \"cpp_threads\": %d}\n",
+                nq / tg, tc > 0 ? nq / tc : -1.0, same, nq, threads);
+    return 0;
+}
+"""
+
+
+def dropin_cpp(ge, cb, ct, ranges, qc, alpha, have_ref):
+    """Build and run DROPIN_CPP (g++, the drop-in header, libfbf.so); None
+    when no C++ compiler is available."""
+    import shutil
+    import subprocess
+    import tempfile
+    if shutil.which("g++") is None:
+        return None
+    lib_dir = os.path.join(ROOT, "sigmod-2024-contest_amd", "lib")
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "dropin.cpp")
+        with open(src, "w") as f:
+            f.write(DROPIN_CPP)
+        exe = os.path.join(d, "dropin")
+        b = subprocess.run(["g++", "-std=c++17", "-O2", "-fopenmp", f"-I{os.path.join(ROOT, 'include')}", src,
+                            f"-L{lib_dir}", "-lfbf", f"-Wl,-rpath,{lib_dir}", "-ldl", "-o", exe],
+                           capture_output=True, text=True)
+        if b.returncode != 0:
+            log(f"drop-in C++ program did not build: {b.stderr[-400:]}")
+            return None
+        files = []
+        for name, arr in (("cb", cb), ("ct", ct), ("q", qc), ("se", np.ascontiguousarray(ranges[:, :3], np.uint64))):
+            path = os.path.join(d, name + ".bin")
+            arr.tofile(path)
+            files.append(path)
+        ref_so = ge.load_oracle().REF_SO if have_ref else "-"
+        r = subprocess.run([exe, *files, repr(float(alpha)), ref_so, "16"], capture_output=True, text=True,
+                           timeout=600)
+        if r.returncode != 0:
+            log(f"drop-in C++ program failed: rc={r.returncode} {r.stderr[-400:]}")
+            return None
+        out = json.loads(r.stdout.strip().splitlines()[-1])
+        log(f"drop-in from C++ (16 OpenMP threads): {out}")
+        return out
 
 
 def committed_traffic(workload, mode, a, kw):

@@ -1697,6 +1697,7 @@ struct ThreadEngine {
     Scratch scr;
     DevBuf tmp_codes;
     HostPinned h_out;
+    uint32_t *done = nullptr;  // arrival counter of sym_one_kernel (zero between calls)
 };
 static std::mutex g_eng_mu;
 static std::vector<ThreadEngine *> *g_eng_free = new std::vector<ThreadEngine *>();  // never destroyed
@@ -1833,6 +1834,48 @@ static fbf_status sym_topk(const int8_t *codes_dev, uint64_t n_rows, const uint6
     if (se != FBF_OK) return se;
     ThreadEngine &E = *Ep;
     hipStream_t st = E.st;
+    if (nq == 1 && counts[0] > 0) {
+        // One query: one launch (the workgroups' lists merged by the last
+        // one to finish) and one copy back -- no plan, no query upload.
+        const uint64_t n = counts[0];
+        if (starts[0] + n > n_rows || n >= 0xFFFFFFFFull) return fail(FBF_ERR_INVALID_ARG, "slice outside the code array");
+        const int B = list_capacity((int)k);
+        // About 1024 rows per workgroup, at most 256 workgroups.
+        uint64_t chunk = std::max<uint64_t>(1024, (n + 255) / 256);
+        chunk = (chunk + kRowTile - 1) / kRowTile * kRowTile;
+        const uint64_t grid = (n + chunk - 1) / chunk;
+        Scratch &S = E.scr;
+        HIP_TRY(S.d_lists.reserve(grid * (uint64_t)B * sizeof(uint64_t)));
+        HIP_TRY(S.d_counts.reserve(grid * sizeof(uint32_t)));
+        HIP_TRY(S.d_out_ids.reserve(2 * (size_t)k * sizeof(uint32_t)));
+        if (!E.done) {
+            HIP_TRY(hipMalloc(&E.done, 256));
+            HIP_TRY(hipMemsetAsync(E.done, 0, 256, st));
+        }
+        SymOneArgs a{};
+        a.codes = codes_dev + starts[0] * 112;
+        a.n = (uint32_t)n;
+        a.chunk = (uint32_t)chunk;
+        a.K = k;
+        std::memcpy(a.q, qcodes_host, 112);
+        a.lists = S.d_lists.as<uint64_t>();
+        a.counts = S.d_counts.as<uint32_t>();
+        a.done = E.done;
+        a.out = S.d_out_ids.as<uint32_t>();
+        HIP_TRY(launch_sym_one(a, B, st));
+        // (Writing the results straight into pinned host memory from the
+        // kernel instead of this copy measured 45% slower per call.)
+        HIP_TRY(E.h_out.reserve(2 * (size_t)k * sizeof(uint32_t)));
+        uint32_t *hp = (uint32_t *)E.h_out.p;
+        HIP_TRY(hipMemcpyAsync(hp, a.out, 2 * (size_t)k * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        for (uint32_t i = 0; i < k; ++i) {
+            const bool pad = hp[i] == kPadId;
+            out_pos[i] = pad ? -1 : (int32_t)hp[i];
+            out_dist[i] = pad ? INT32_MAX : (int32_t)hp[k + i];
+        }
+        return FBF_OK;
+    }
     Plan P;
     fbf_status s = sym_plan(starts, counts, nq, n_rows, P);
     if (s != FBF_OK) return s;

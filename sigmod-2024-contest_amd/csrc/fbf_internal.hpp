@@ -229,6 +229,23 @@ hipError_t launch_gather_rows(const float *src, const uint32_t *idx, uint64_t n,
 hipError_t launch_scatter_results(const uint32_t *ids, const float *d, const uint32_t *idx, uint64_t n, uint32_t k,
                                   uint32_t *out_ids, float *out_d, hipStream_t st);
 hipError_t launch_scan_sq8sym(const ScanSymArgs &a, int B, hipStream_t st);
+// One query over one slice in a single launch (the per-call drop-in): each
+// workgroup scans `chunk` rows of the slice into its own list; the last to
+// finish (a device counter, zero between calls and reset by that workgroup)
+// merges the lists and writes out[0..K) = slice positions (kPadId pad) and
+// out[K..2K) = int distances (INT32_MAX pad), ascending (distance, position).
+struct SymOneArgs {
+    const int8_t *codes;   // the slice's first row (112-byte rows)
+    uint32_t n;            // rows in the slice
+    uint32_t chunk;        // rows per workgroup (a multiple of kRowTile)
+    uint32_t K;
+    int4 q[7];             // the query's 112 code bytes
+    uint64_t *lists;       // [gridDim.x][B]
+    uint32_t *counts;      // [gridDim.x]
+    uint32_t *done;        // arrival counter
+    uint32_t *out;         // [2K]
+};
+hipError_t launch_sym_one(const SymOneArgs &a, int B, hipStream_t st);
 // variant: 0 BF16, 1 FP8, 2 SQ8, 3 symmetric int8; B: list capacity (128,
 // 256, 512, 1024; at most a.list_stride); nw: waves per workgroup (1, 2, 4),
 // an item carries at most 64 * nw queries.  launch_build_mfma with

@@ -148,6 +148,89 @@ hipError_t launch_scan_sq8sym(const ScanSymArgs &a, int B, hipStream_t st) {
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------- one query, one launch
+
+template <int B>
+__global__ __launch_bounds__(64) void sym_one_kernel(SymOneArgs a) {
+    __shared__ uint64_t sbuf[B];
+    __shared__ int4 stile[kRowTile * kCodeInt4];
+    __shared__ uint32_t s_last;
+    const uint32_t lane = threadIdx.x;
+    int4 q[kCodeInt4];
+#pragma unroll
+    for (int c = 0; c < kCodeInt4; ++c) q[c] = a.q[c];
+    const int qq = dot_codes(q, q);
+    const uint32_t r0 = blockIdx.x * a.chunk;
+    const uint32_t r1 = min(r0 + a.chunk, a.n);
+    uint64_t *list = a.lists + (uint64_t)blockIdx.x * B;
+    uint32_t cnt = 0;
+    uint64_t tau = kKeyMax;
+    const int4 *src = reinterpret_cast<const int4 *>(a.codes);
+#pragma unroll 1
+    for (uint32_t t = r0; t < r1; t += kRowTile) {
+        // Coalesced tile load: int4 number (t*7 + lane + 64c) of the slice.
+#pragma unroll
+        for (int c = 0; c < kCodeInt4; ++c) {
+            const uint32_t e = lane + 64 * c;
+            const uint32_t row = t + e / kCodeInt4;
+            stile[e] = row < r1 ? src[(uint64_t)t * kCodeInt4 + e] : make_int4(0, 0, 0, 0);
+        }
+        __syncthreads();
+        int4 x[kCodeInt4];
+#pragma unroll
+        for (int c = 0; c < kCodeInt4; ++c) x[c] = stile[lane * kCodeInt4 + c];
+        __syncthreads();
+        const uint32_t row = t + lane;
+        const int dist = dot_codes(x, x) - 2 * dot_codes(x, q) + qq;
+        const uint64_t key = ((uint64_t)(uint32_t)dist << 32) | row;
+        offer_global<B>(list, cnt, tau, a.K, sbuf, row < r1 && key < tau, key);
+    }
+    if (lane == 0) a.counts[blockIdx.x] = cnt;
+    // Publish this list, then count arrivals; the last workgroup merges.
+    __threadfence();
+    if (lane == 0) {
+        const uint32_t old = __hip_atomic_fetch_add(a.done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = old == gridDim.x - 1 ? 1u : 0u;
+    }
+    __syncthreads();
+    if (s_last == 0) return;
+    __threadfence();
+    uint32_t mc = 0;
+    uint64_t mt = kKeyMax;
+#pragma unroll 1
+    for (uint32_t g = 0; g < gridDim.x; ++g) {
+        const uint32_t c = __hip_atomic_load(&a.counts[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t *L = a.lists + (uint64_t)g * B;
+#pragma unroll 1
+        for (uint32_t i = 0; i < c; i += 64) {
+            const uint32_t ii = i + lane;
+            const uint64_t key = ii < c ? __hip_atomic_load(&L[ii], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                        : kKeyMax;
+            offer_lds<B>(sbuf, mc, mt, a.K, key < mt, key);
+        }
+    }
+    compact_lds<B>(sbuf, mc, mt, a.K);  // ascending, mc <= K
+    for (uint32_t i = lane; i < a.K; i += 64) {
+        const bool v = i < mc;
+        a.out[i] = v ? (uint32_t)sbuf[i] : kPadId;
+        a.out[a.K + i] = v ? (uint32_t)(sbuf[i] >> 32) : 0x7FFFFFFFu;
+    }
+    if (lane == 0) __hip_atomic_store(a.done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+hipError_t launch_sym_one(const SymOneArgs &a, int B, hipStream_t st) {
+    if (a.n == 0 || a.chunk == 0 || a.chunk % kRowTile != 0) return hipErrorInvalidValue;
+    if ((int)a.K + kRowTile > B) return hipErrorInvalidValue;
+    const uint32_t grid = (a.n + a.chunk - 1) / a.chunk;
+    switch (B) {
+    case 256: hipLaunchKernelGGL(sym_one_kernel<256>, dim3(grid), dim3(64), 0, st, a); break;
+    case 512: hipLaunchKernelGGL(sym_one_kernel<512>, dim3(grid), dim3(64), 0, st, a); break;
+    case 1024: hipLaunchKernelGGL(sym_one_kernel<1024>, dim3(grid), dim3(64), 0, st, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 }  // namespace fbf
 
 namespace fbf {
