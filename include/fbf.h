@@ -88,7 +88,7 @@ typedef struct fbf_search_opts {
     uint32_t prepass_div; /* MFMA modes: a prefix pass over the first 1/prepass_div of each large
                              range seeds the per-query thresholds (0 -> default 16; 16 for EXACT_BF16; 1 -> off) */
     uint32_t prepass_margin; /* percent of k'/prepass_div keys the prefix pass keeps; its last
-                                kept key is the seed (0 -> default 300; 200 for EXACT_BF16).  Queries whose seed proves
+                                kept key is the seed (0 -> 300 for k' <= 160, 200 up to 500, else 150; 200 for EXACT_BF16).  Queries whose seed proves
                                 too tight are rescanned, so results never depend on it. */
     uint32_t max_batch;   /* fbf_search / fbf_search_host: larger batches run as consecutive
                              sub-batches of this many queries (0 -> 131072), which bounds the

@@ -915,11 +915,14 @@ static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, 
     // thousands an unseeded scan appends while its lists fill.  The merge
     // verifies each seeded query (exact when its k'-th merged key is <= tau0)
     // and the few that fail are rescanned over their whole range.
-    // Defaults 1/16 and 300%: on C1 (BF16, k' = 110) 88.3 ms per step against
-    // 89.9 with 1/10 and 250%, and no seed fails; 1/24 or 1/32 with 250-300%
-    // let seeds fail (rescans), and 400% appends more than it saves.
+    // Defaults 1/16 and a margin that shrinks as k' grows: the seed estimates
+    // the (margin * k')-th key from the kpre-th key of a 1/16 sample, whose
+    // relative spread falls as 1/sqrt(kpre).  C1: BF16 k' = 110 at 300%
+    // 88.3 ms per step (89.9 with 1/10 and 250%; 1/24 or 1/32 let seeds
+    // fail); k' = 400 at 200% 104.4 ms (300%: 111.0; 150%: seeds fail, 132.7);
+    // FP8 k' = 896 at 150% 148.5 ms (200%: 173.4; 300%: 219.9).
     const uint32_t pdiv = mfma ? (o.prepass_div ? o.prepass_div : 16u) : 1u;
-    const uint32_t margin = o.prepass_margin ? o.prepass_margin : 300u;
+    const uint32_t margin = o.prepass_margin ? o.prepass_margin : K <= 160 ? 300u : K <= 500 ? 200u : 150u;
     const uint32_t kpre = (uint32_t)std::min<uint64_t>(K, std::max<uint64_t>(1, ((uint64_t)K * margin + 100ull * pdiv - 1) / (100ull * pdiv)));
     Plan P;
     uint64_t n_items_a = 0;
