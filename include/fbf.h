@@ -95,9 +95,11 @@ typedef struct fbf_search_opts {
                              candidate-list memory (about 0.25 MB per query at the BF16
                              defaults on 10 M rows); results do not depend on it.  Captured
                              searches (fbf_search_graph_create) are not split. */
-    uint32_t metas_on_device; /* fbf_search only: 1 -> metas is DEVICE memory [nq][4]; the
-                                 ranges come from the device router (fbf_route) and the host
-                                 plans from them.  Results equal those of host metas. */
+    uint32_t metas_on_device; /* fbf_search: 1 -> metas is DEVICE memory [nq][4]; the ranges
+                                 come from the device router (fbf_route) and the host plans
+                                 from them.  Results equal those of host metas.
+                                 fbf_search_graph_create: 1 -> an any-batch graph that reads
+                                 metas at every replay and plans on the device. */
 } fbf_search_opts;
 
 /* Create a device-resident dataset on the current HIP device from host
@@ -158,7 +160,21 @@ fbf_status fbf_search_host(fbf_dataset *ds, const float *queries, const float *m
  * caller may refill the query vectors between replays (the filters, metas,
  * are fixed by the plan).  Replays on one graph must not overlap.  Searches
  * on the dataset may run meanwhile (the graph has its own scratch).  The
- * rescan count of fbf_last_search_rescans is not recorded for replays. */
+ * rescan count of fbf_last_search_rescans is not recorded for replays.
+ *
+ * Any-batch form (opts->metas_on_device = 1, modes BF16, FP8, SQ8, SQ8SYM):
+ * metas is DEVICE memory [nq][4] and is read at every replay, like the
+ * queries: the graph holds the device router (fbf_route's range step), a
+ * planner that groups the queries and cuts their ranges into work items on
+ * the device, and the seeded search, so each replay serves a new batch of nq
+ * filtered queries with no host work.  Results equal fbf_search's on the same
+ * queries and metas (the plans differ; the kept keys do not).  Unknown filter
+ * types and empty ranges give padded outputs (id 0xFFFFFFFF, distance
+ * +inf).  Buffers are sized at creation for the worst plan of nq queries,
+ * the candidate lists at most 32 GB; a replay whose plan needs more list
+ * slots than that returns padded outputs for every query and sets the
+ * overflow flag of fbf_search_graph_plan_info (re-run such a batch with
+ * fbf_search or with larger chunk_rows). */
 typedef struct fbf_search_graph fbf_search_graph;
 fbf_status fbf_search_graph_create(fbf_dataset *ds, const float *queries_dev, const float *metas,
                                    uint64_t nq, uint32_t k, const fbf_search_opts *opts,
@@ -166,6 +182,12 @@ fbf_status fbf_search_graph_create(fbf_dataset *ds, const float *queries_dev, co
 fbf_status fbf_search_graph_launch(fbf_search_graph *g, void *stream);
 /* Waits for the device, then frees the graph and its scratch. */
 fbf_status fbf_search_graph_destroy(fbf_search_graph *g);
+/* Any-batch graphs: the plan of the last completed replay (the caller waits
+ * for it first): work items of the prefix and main passes and the overflow
+ * flag (1: the plan exceeded the graph's capacity, outputs padded).  No
+ * counterpart in the reference (its plan is the OpenMP loop over queries). */
+fbf_status fbf_search_graph_plan_info(fbf_search_graph *g, uint64_t *items_prefix, uint64_t *items_main,
+                                      uint32_t *overflow);
 
 /* Type-2 sub-interval search: the SMALL-bucket branch of the router's
  * type-2 path (hybrid_graph.cpp:317-420) and the global refine (:463-494) for

@@ -1406,12 +1406,269 @@ extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, cons
 }
 
 // ---------------------------------------------------------------- captured searches
+// Buffers of a plan built on the device (any-batch graphs).
+struct DPlanBufs {
+    DevBuf ranges, rkeys, rvals, keys, skeys, vals, order, hist, hoff, gd, gnc, qnc, ioff_pre, ioff_main,
+        items_tmp_pre, items_tmp_main, items_pre, items_main, islot_pre, islot_main, bins, boff, iq_pre, iq_main,
+        n_items, xoff, status;
+    void release() {
+        for (DevBuf *b : {&ranges, &rkeys, &rvals, &keys, &skeys, &vals, &order, &hist, &hoff, &gd, &gnc, &qnc,
+                          &ioff_pre, &ioff_main, &items_tmp_pre, &items_tmp_main, &items_pre, &items_main, &islot_pre,
+                          &islot_main, &bins, &boff, &iq_pre, &iq_main, &n_items, &xoff, &status})
+            b->release();
+    }
+};
+
 struct fbf_search_graph {
     fbf_dataset *ds = nullptr;
     Scratch scr;      // the graph's own plan, lists and thresholds
     GraphOut g;
     uint64_t nq = 0;
+    bool any_batch = false;  // plans on the device from metas read at each replay
+    DPlanBufs dp;
 };
+
+// Over-fetch depth of an approximate search (0 in o.kprime: the mode's default).
+static fbf_status kprime_of(const fbf_search_opts &o, uint32_t k, uint32_t &K) {
+    const bool fine = o.mode == FBF_MODE_BF16 || o.mode == FBF_MODE_SQ8;
+    K = o.kprime ? o.kprime : std::min<uint32_t>(fine ? (3 * k + 1) / 2 : o.mode == FBF_MODE_FP8 ? 9 * k : 4 * k, 896);
+    if (K < k) return fail(FBF_ERR_INVALID_ARG, "kprime must be >= k");
+    if (K > 896) return fail(FBF_ERR_INVALID_ARG, "kprime must be <= 896");
+    return FBF_OK;
+}
+
+// Candidate-list memory of a graph that plans on the device: room for every
+// query's lists at the worst plan, at most this many bytes.
+constexpr uint64_t kAnyBatchListBytes = 32ull << 30;
+
+// An any-batch graph: the router kernel, the device planner (fbf_dplan.hip)
+// and the seeded MFMA search captured together, so each replay reads the
+// filters (metas, device memory) and the vectors of that moment.  Buffers are
+// sized at creation: work items for the worst plan of nq queries over this
+// dataset (each query its own group over a whole layout), list slots for the
+// worst plan or the list-memory cap, whichever is smaller.  A replay whose
+// plan needs more slots returns padded outputs and sets the overflow flag
+// fbf_search_graph_plan_info reads.
+static fbf_status capture_any_batch(fbf_dataset *ds, fbf_search_graph *G, const float *queries_dev,
+                                    const float *metas_dev, uint64_t nq, uint32_t k, const fbf_search_opts &o,
+                                    uint32_t *out_ids_dev, float *out_dists_dev, hipStream_t st) {
+    if (!queries_dev || !metas_dev || !out_ids_dev || !out_dists_dev) return fail(FBF_ERR_INVALID_ARG, "null argument");
+    if (k == 0 || k > 512) return fail(FBF_ERR_INVALID_ARG, "k must be in [1, 512]");
+    const bool mfma = o.mode == FBF_MODE_BF16 || o.mode == FBF_MODE_FP8 || o.mode == FBF_MODE_SQ8 ||
+                      o.mode == FBF_MODE_SQ8SYM;
+    if (!mfma) return fail(FBF_ERR_INVALID_ARG, "graphs that plan on the device run the BF16, FP8, SQ8 and SQ8SYM modes");
+    fbf_status s = mode_check(ds, o.mode);
+    if (s != FBF_OK) return s;
+    if (nq >= 0x7FFFFFFFull) return fail(FBF_ERR_INVALID_ARG, "batch too large");
+    uint32_t K = 0;
+    s = kprime_of(o, k, K);
+    if (s != FBF_OK) return s;
+    const uint32_t chunk = o.chunk_rows ? (o.chunk_rows + kRowTile - 1) / kRowTile * kRowTile : 65536u;
+    const uint32_t gmax = o.group_queries ? std::min<uint32_t>(o.group_queries, 256u) : 256u;
+    const uint32_t pdiv = o.prepass_div ? o.prepass_div : 16u;
+    const uint32_t margin = o.prepass_margin ? o.prepass_margin : K <= 160 ? 300u : K <= 500 ? 200u : 150u;
+    const uint32_t kpre = (uint32_t)std::min<uint64_t>(
+        K, std::max<uint64_t>(1, ((uint64_t)K * margin + 100ull * pdiv - 1) / (100ull * pdiv)));
+    const bool seeded = pdiv > 1;
+    const int B = K <= 160 ? 256 : K <= 300 ? 512 : 1024;
+    const int B_pre = !seeded ? B : std::min(B, kpre <= 64 ? 128 : kpre <= 160 ? 256 : kpre <= 300 ? 512 : 1024);
+    const int nw = gmax <= kMfmaWaveQueries ? 1 : gmax <= 2 * kMfmaWaveQueries ? 2 : 4;
+    // Worst case: every query its own group over a whole layout.
+    const uint64_t nchunk = (ds->n + kRowTile + chunk - 1) / chunk + 1;
+    const uint64_t max_items = nq * nchunk;
+    const uint64_t max_slots =
+        std::min<uint64_t>({2 * nq * nchunk, kAnyBatchListBytes / ((uint64_t)B * sizeof(uint64_t)), 0xFFFFFFF0ull});
+    if (max_items >= 0xFFFFFFF0ull) return fail(FBF_ERR_INVALID_ARG, "batch too large for a graph that plans on the device");
+    const uint64_t nwin = (nq + gmax - 1) / gmax, ngslot = nwin * gmax;
+    const uint64_t nblk = (nq + 255) / 256;
+    const uint32_t nbins = dplan_bins(ds->n, chunk);
+    Scratch &S = G->scr;
+    DPlanBufs &D = G->dp;
+    HIP_TRY(hipSetDevice(ds->device));
+    HIP_TRY(D.ranges.reserve(nq * 4 * sizeof(uint32_t)));
+    HIP_TRY(D.rkeys.reserve(nq * 4 * sizeof(uint64_t)));
+    HIP_TRY(D.rvals.reserve(nq * 2 * sizeof(uint32_t)));
+    for (DevBuf *b : {&D.keys, &D.skeys}) HIP_TRY(b->reserve(nq * sizeof(uint64_t)));
+    for (DevBuf *b : {&D.vals, &D.order}) HIP_TRY(b->reserve(nq * sizeof(uint32_t)));
+    HIP_TRY(D.hist.reserve(256 * nblk * sizeof(uint32_t)));
+    HIP_TRY(D.hoff.reserve((256 * nblk + 1) * sizeof(uint64_t)));
+    HIP_TRY(D.gd.reserve(ngslot * 8 * sizeof(uint32_t)));
+    HIP_TRY(D.gnc.reserve(ngslot * 2 * sizeof(uint32_t)));
+    HIP_TRY(D.qnc.reserve(nq * 2 * sizeof(uint32_t)));
+    HIP_TRY(D.ioff_pre.reserve((ngslot + 1) * sizeof(uint64_t)));
+    HIP_TRY(D.ioff_main.reserve((ngslot + 1) * sizeof(uint64_t)));
+    for (DevBuf *b : {&D.items_tmp_pre, &D.items_tmp_main, &D.items_pre, &D.items_main})
+        HIP_TRY(b->reserve(max_items * sizeof(WorkItem)));
+    for (DevBuf *b : {&D.islot_pre, &D.islot_main}) HIP_TRY(b->reserve(max_items * sizeof(uint32_t)));
+    HIP_TRY(D.bins.reserve(2ull * nbins * sizeof(uint32_t)));
+    HIP_TRY(D.boff.reserve(2ull * (nbins + 1) * sizeof(uint64_t)));
+    HIP_TRY(D.iq_pre.reserve(nq * sizeof(ItemQuery)));
+    HIP_TRY(D.iq_main.reserve(nq * sizeof(ItemQuery)));
+    HIP_TRY(D.n_items.reserve(2 * sizeof(uint64_t)));
+    HIP_TRY(D.xoff.reserve(18 * sizeof(uint32_t)));
+    HIP_TRY(D.status.reserve(4 * sizeof(uint32_t)));
+    HIP_TRY(S.d_slot_off.reserve((nq + 1) * sizeof(uint64_t)));
+    HIP_TRY(S.d_pre_end.reserve(nq * sizeof(uint64_t)));
+    HIP_TRY(S.d_lists.reserve(max_slots * (size_t)B * sizeof(uint64_t)));
+    HIP_TRY(S.d_counts.reserve(max_slots * sizeof(uint32_t)));
+    HIP_TRY(S.d_queue.reserve(8 * 64));
+    HIP_TRY(S.d_gtau.reserve(nq * sizeof(uint64_t)));
+    HIP_TRY(S.d_tau0.reserve(nq * sizeof(uint64_t)));
+    HIP_TRY(S.d_flags.reserve((nq + 1) * sizeof(uint32_t)));
+    HIP_TRY(S.d_qf.reserve(nq * kQFragU4 * sizeof(uint4)));
+    if (o.mode == FBF_MODE_SQ8SYM) HIP_TRY(S.d_q.reserve(nq * 112));
+
+    DPlan dp{};
+    dp.ranges = D.ranges.as<uint32_t>();
+    dp.nq = nq;
+    dp.n_rows = ds->n;
+    dp.gmax = gmax;
+    dp.chunk = chunk;
+    dp.pdiv = seeded ? pdiv : 1u;
+    dp.min_prefix = (uint32_t)std::max<uint64_t>(kMinPrefixRows, 2ull * kpre);
+    dp.max_items = max_items;
+    dp.max_slots = max_slots;
+    dp.keys = D.keys.as<uint64_t>();
+    dp.skeys = D.skeys.as<uint64_t>();
+    dp.vals = D.vals.as<uint32_t>();
+    dp.order = D.order.as<uint32_t>();
+    dp.hist = D.hist.as<uint32_t>();
+    dp.hoff = D.hoff.as<uint64_t>();
+    dp.gd = D.gd.as<uint32_t>();
+    dp.gnc = D.gnc.as<uint32_t>();
+    dp.qnc = D.qnc.as<uint32_t>();
+    dp.ioff_pre = D.ioff_pre.as<uint64_t>();
+    dp.ioff_main = D.ioff_main.as<uint64_t>();
+    dp.slot_off = S.d_slot_off.as<uint64_t>();
+    dp.pre_end = S.d_pre_end.as<uint64_t>();
+    dp.items_tmp_pre = D.items_tmp_pre.as<WorkItem>();
+    dp.items_tmp_main = D.items_tmp_main.as<WorkItem>();
+    dp.items_pre = D.items_pre.as<WorkItem>();
+    dp.items_main = D.items_main.as<WorkItem>();
+    dp.islot_pre = D.islot_pre.as<uint32_t>();
+    dp.islot_main = D.islot_main.as<uint32_t>();
+    dp.bins = D.bins.as<uint32_t>();
+    dp.boff = D.boff.as<uint64_t>();
+    dp.iq_pre = D.iq_pre.as<ItemQuery>();
+    dp.iq_main = D.iq_main.as<ItemQuery>();
+    dp.n_items = D.n_items.as<uint64_t>();
+    dp.xoff_pre = D.xoff.as<uint32_t>();
+    dp.xoff_main = D.xoff.as<uint32_t>() + 9;
+    dp.status = D.status.as<uint32_t>();
+
+    const int variant = o.mode == FBF_MODE_BF16 ? 0 : o.mode == FBF_MODE_FP8 ? 1 : o.mode == FBF_MODE_SQ8 ? 2 : 3;
+    warm_scan_mfma(variant, B, nw);
+    if (seeded) warm_scan_mfma(variant, B_pre, nw);
+    HIP_TRY(hipMemsetAsync(D.status.p, 0, 4 * sizeof(uint32_t), st));  // before the capture: no memset node
+    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    fbf_status sw = [&]() -> fbf_status {
+        HIP_TRY(launch_route(metas_dev, nq, ds->d_lab_b, ds->d_ts_b, ds->d_ts_t, ds->n, (float)ds->n, 0.045f, 0.08f,
+                             D.ranges.as<uint32_t>(), nullptr, D.rkeys.as<uint64_t>(), D.rvals.as<uint32_t>(), nullptr,
+                             0, st));
+        HIP_TRY(launch_fill_u32(dp.status, 0u, 4, st));
+        HIP_TRY(launch_dplan(dp, st));
+        uint64_t *gtau = S.d_gtau.as<uint64_t>();
+        uint32_t *flags = S.d_flags.as<uint32_t>();
+        uint32_t *queue = S.d_queue.as<uint32_t>();
+        HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t *>(gtau), 0xFFFFFFFFu, 2 * nq, st));
+        HIP_TRY(launch_fill_u32(flags, 0u, nq + 1, st));
+        if (variant == 3) HIP_TRY(launch_sym_encode(queries_dev, nullptr, nq, nq, ds->alpha, S.d_q.as<int8_t>(), st));
+        HIP_TRY(launch_qfrag(variant, queries_dev, ds->sq8_w, variant == 3 ? S.d_q.as<int8_t>() : nullptr, nq,
+                             S.d_qf.as<uint4>(), st));
+        ScanMfmaArgs a{};
+        a.qfrag = S.d_qf.as<uint4>();
+        for (int L = 0; L < 2; ++L) {
+            const LayoutDev &Ld = ds->lay[L];
+            a.lay[L].ids = Ld.ids;
+            a.lay[L].tiles = variant == 0 ? Ld.bf16 : variant == 1 ? Ld.fp8 : variant == 2 ? Ld.sq8 : Ld.sym_mfma;
+            a.lay[L].xnorm = variant == 1 ? Ld.xnorm : variant == 2 ? Ld.sq8_norm : Ld.sym_norm;
+            a.lay[L].scale = Ld.fp8_scale;
+        }
+        a.lists = S.d_lists.as<uint64_t>();
+        a.counts = S.d_counts.as<uint32_t>();
+        a.gtau = gtau;
+        a.list_stride = (uint32_t)B;
+        a.queue = queue;
+        a.n_items = max_items;  // grid sizing; the runs come from the device
+        auto scan = [&](int pass, uint32_t kk, int cap_b, const uint32_t *mask) -> fbf_status {
+            a.items = pass ? dp.items_main : dp.items_pre;
+            a.iq = pass ? dp.iq_main : dp.iq_pre;
+            a.xoff_dev = pass ? dp.xoff_main : dp.xoff_pre;
+            a.K = kk;
+            a.qmask = mask;
+            a.nmask = mask ? mask + nq : nullptr;
+            HIP_TRY(launch_fill_u32(queue, 0u, 8 * 16, st));
+            HIP_TRY(launch_scan_mfma(variant, a, cap_b, nw, st));
+            return FBF_OK;
+        };
+        MergeArgs m{};
+        m.slot_off = S.d_slot_off.as<uint64_t>();
+        m.lists = S.d_lists.as<uint64_t>();
+        m.counts = S.d_counts.as<uint32_t>();
+        m.list_cap = (uint32_t)B;
+        m.nq = nq;
+        m.k = k;
+        m.kprime = K;
+        m.queries = queries_dev;
+        m.base_rows = ds->base_rows;
+        m.out_ids = out_ids_dev;
+        m.out_dists = out_dists_dev;
+        m.rerank = 1;
+        m.key_pos = 1;
+        m.ids_lay[0] = ds->lay[0].ids;
+        m.ids_lay[1] = ds->lay[1].ids;
+        fbf_status r = FBF_OK;
+        if (!seeded) {
+            r = scan(1, K, B, nullptr);
+            if (r != FBF_OK) return r;
+            HIP_TRY(launch_merge(m, st));
+            return FBF_OK;
+        }
+        r = scan(0, kpre, B_pre, nullptr);
+        if (r != FBF_OK) return r;
+        m.tau0 = S.d_tau0.as<uint64_t>();
+        m.flags = flags;
+        m.gtau = gtau;
+        m.nflag = flags + nq;
+        m.pre_end = S.d_pre_end.as<uint64_t>();
+        m.pass = 3;  // seeds: the kpre-th key of each query's prefix lists
+        m.kprime = kpre;
+        HIP_TRY(launch_merge(m, st));
+        m.kprime = K;
+        r = scan(1, K, B, nullptr);
+        if (r != FBF_OK) return r;
+        m.pass = 1;  // verify each seeded query
+        HIP_TRY(launch_merge(m, st));
+        r = scan(0, K, B, flags);  // rescan the failures over their whole range
+        if (r != FBF_OK) return r;
+        r = scan(1, K, B, flags);
+        if (r != FBF_OK) return r;
+        m.pass = 2;
+        HIP_TRY(launch_merge(m, st));
+        return FBF_OK;
+    }();
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    if (sw != FBF_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return sw;
+    }
+    if (e != hipSuccess || g == nullptr) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(FBF_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    }
+    hipGraphExec_t x = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(FBF_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    }
+    G->g.graph = g;
+    G->g.exec = x;
+    G->any_batch = true;
+    return FBF_OK;
+}
 
 extern "C" fbf_status fbf_search_graph_create(fbf_dataset *ds, const float *queries_dev, const float *metas,
                                               uint64_t nq, uint32_t k, const fbf_search_opts *opts,
@@ -1428,8 +1685,11 @@ extern "C" fbf_status fbf_search_graph_create(fbf_dataset *ds, const float *quer
     hipStream_t st = nullptr;
     hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
     fbf_status s = e == hipSuccess ? FBF_OK : fail(FBF_ERR_DEVICE, "hipStreamCreate");
-    if (s == FBF_OK)
+    if (s == FBF_OK && opts && opts->metas_on_device) {
+        s = capture_any_batch(ds, G, queries_dev, metas, nq, k, *opts, out_ids_dev, out_dists_dev, st);
+    } else if (s == FBF_OK) {
         s = search_impl(ds, G->scr, false, queries_dev, metas, nq, k, opts, out_ids_dev, out_dists_dev, st, &G->g);
+    }
     if (st) {
         (void)hipStreamSynchronize(st);
         (void)hipStreamDestroy(st);
@@ -1458,7 +1718,21 @@ extern "C" fbf_status fbf_search_graph_destroy(fbf_search_graph *g) {
     if (g->g.exec) (void)hipGraphExecDestroy(g->g.exec);
     if (g->g.graph) (void)hipGraphDestroy(g->g.graph);
     g->scr.release();
+    g->dp.release();
     delete g;
+    return FBF_OK;
+}
+
+extern "C" fbf_status fbf_search_graph_plan_info(fbf_search_graph *g, uint64_t *items_prefix, uint64_t *items_main,
+                                                 uint32_t *overflow) {
+    if (!g) return fail(FBF_ERR_INVALID_ARG, "null graph");
+    if (!g->any_batch) return fail(FBF_ERR_INVALID_ARG, "the graph was planned on the host");
+    HIP_TRY(hipSetDevice(g->ds->device));
+    uint32_t st[4] = {0, 0, 0, 0};
+    HIP_TRY(hipMemcpy(st, g->dp.status.p, sizeof(st), hipMemcpyDeviceToHost));
+    if (items_prefix) *items_prefix = st[0];
+    if (items_main) *items_main = st[1];
+    if (overflow) *overflow = st[2];
     return FBF_OK;
 }
 

@@ -148,6 +148,7 @@ struct ScanMfmaArgs {
     // own XCD's run (counter queue[16 x]) and then from the others' runs.
     uint32_t *queue;
     uint32_t xoff[9];
+    const uint32_t *xoff_dev;  // null, or the runs in device memory (a plan built on the device)
     const uint32_t *qmask;   // [nq] or null: when set, only queries with qmask[qi] != 0 are scanned
     const uint32_t *nmask;   // null, or the number of queries set in qmask: the launch exits when 0
 };
@@ -252,6 +253,38 @@ hipError_t launch_sym_one(const SymOneArgs &a, int B, hipStream_t st);
 // variant 3 reads the layout's symmetric codes from sq8_codes and writes the
 // code norms to norms.
 hipError_t launch_scan_mfma(int variant, const ScanMfmaArgs &a, int B, int nw, hipStream_t st);
+// The seeded MFMA search's plan built on the device from ranges in device
+// memory (fbf_dplan.hip).  All pointers are device buffers of the stated
+// capacity; n_items[0] / [1] receive the item counts of the two passes,
+// status = {items of the prefix pass, items of the main pass, overflow}.
+struct DPlan {
+    const uint32_t *ranges;  // [nq][4] {layout, s, e, -}
+    uint64_t nq;
+    uint64_t n_rows;         // rows of each layout
+    uint32_t gmax, chunk, pdiv, min_prefix;
+    uint64_t max_items, max_slots;
+    uint64_t *keys, *skeys;        // [nq] (ping-pong buffers of the query sort)
+    uint32_t *vals, *order;        // [nq]
+    uint32_t *hist;                // [256 * ceil(nq / 256)]
+    uint64_t *hoff;                // [256 * ceil(nq / 256) + 1]
+    uint32_t *gd;                  // [ceil(nq / gmax) * gmax][8]
+    uint32_t *gnc;                 // [ceil(nq / gmax) * gmax][2]
+    uint32_t *qnc;                 // [nq][2]
+    uint64_t *ioff_pre, *ioff_main;  // [ceil(nq / gmax) * gmax + 1]
+    uint64_t *slot_off;            // [nq + 1]
+    uint64_t *pre_end;             // [nq]
+    WorkItem *items_tmp_pre, *items_tmp_main, *items_pre, *items_main;  // [max_items] each
+    uint32_t *islot_pre, *islot_main;  // [max_items] each: an item's place in its row bin
+    uint32_t *bins;                // [2][dplan_bins(n_rows, chunk)]: items per row bin
+    uint64_t *boff;                // [2][dplan_bins(n_rows, chunk) + 1]
+    ItemQuery *iq_pre, *iq_main;   // [nq] each (indexed by sorted position)
+    uint64_t *n_items;             // [2]
+    uint32_t *xoff_pre, *xoff_main;  // [9] each
+    uint32_t *status;              // [4]
+};
+// Row bins of the item order: both layouts' rows cut at multiples of chunk.
+inline uint32_t dplan_bins(uint64_t n_rows, uint32_t chunk) { return (uint32_t)(2 * (n_rows / chunk + 1)); }
+hipError_t launch_dplan(const DPlan &dp, hipStream_t st);
 // B-operand fragments of a batch's queries for the MFMA scans, kQFragU4
 // uint4 per query (variant 3 reads the queries' symmetric int8 codes).
 constexpr int kQFragU4 = 16;

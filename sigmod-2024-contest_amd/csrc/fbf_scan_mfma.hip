@@ -1012,14 +1012,15 @@ void scan_mfma_kernel(ScanMfmaArgs a) {
             if (a.queue == nullptr) {
                 idx = round == 0 ? blockIdx.x : kPadId;
             } else {
+                const uint32_t *xo = a.xoff_dev ? a.xoff_dev : a.xoff;
                 for (uint32_t t = 0; t < 8; ++t) {
                     const uint32_t x = (me + t) & 7u;
-                    const uint32_t n = a.xoff[x + 1] - a.xoff[x];
+                    const uint32_t n = xo[x + 1] - xo[x];
                     if (n == 0) continue;
                     if (__hip_atomic_load(&a.queue[16 * x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= n) continue;
                     const uint32_t i = atomicAdd(&a.queue[16 * x], 1u);
                     if (i < n) {
-                        idx = a.xoff[x] + i;
+                        idx = xo[x] + i;
                         break;
                     }
                 }

@@ -42,6 +42,7 @@ EXPORTS = [
     "fbf_last_error", "fbf_version", "fbf_search_graph_create", "fbf_search_graph_launch",
     "fbf_search_graph_destroy", "fbf_last_search_fallbacks", "fbf_dataset_layout_rows",
     "fbf_last_search_pass_pairs", "fbf_bruteforce_batch_noalloc", "fbf_search_subinterval", "fbf_route",
+    "fbf_search_graph_plan_info",
 ]
 
 
@@ -103,6 +104,8 @@ def lib() -> C.CDLL:
                                           C.POINTER(_P)]
     L.fbf_search_graph_launch.argtypes = [_P, _P]
     L.fbf_search_graph_destroy.argtypes = [_P]
+    L.fbf_search_graph_plan_info.argtypes = [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                             C.POINTER(C.c_uint32)]
     L.fbf_dataset_layout_rows.argtypes = [_P, C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint32, _P, _P]
     for name in EXPORTS:
         if name not in ("fbf_last_error", "fbf_version"):
@@ -400,26 +403,45 @@ def bruteforce_batch_noalloc(codes_base: np.ndarray, starts, counts, qcodes: np.
 
 class SearchGraph:
     """One batch's search captured as a HIP graph (fbf_search_graph_create):
-    ``launch()`` replays it, reading the query tensor given at creation (its
-    contents may change between replays; the filters may not) and writing
-    ``ids`` / ``dists``."""
+    ``launch()`` replays it, reading the query tensor given at creation and
+    writing ``ids`` / ``dists``.  With metas as a numpy array the plan is made
+    once on the host (the query vectors may change between replays, the
+    filters may not); with metas as a torch CUDA tensor the graph routes and
+    plans on the device at every replay, so both tensors may be refilled
+    (BF16, FP8, SQ8, SQ8SYM modes)."""
 
-    def __init__(self, ds: "Dataset", queries, metas: np.ndarray, k: int = 100, mode="bf16", kprime: int = 0,
+    def __init__(self, ds: "Dataset", queries, metas, k: int = 100, mode="bf16", kprime: int = 0,
                  chunk_rows: int = 0, group_queries: int = 0, persistent: int = 0, prepass_div: int = 0,
                  prepass_margin: int = 0):
         import torch
-        metas = np.ascontiguousarray(metas, np.float32)
+        self.any_batch = isinstance(metas, torch.Tensor)
+        if self.any_batch:
+            assert metas.is_cuda and metas.dtype == torch.float32 and metas.is_contiguous()
+            self.metas = metas  # kept alive: every replay reads it
+            mp = _P(_dev_ptr(metas))
+        else:
+            metas = np.ascontiguousarray(metas, np.float32)
+            mp = _np_ptr(metas)
         nq = metas.shape[0]
         assert queries.shape == (nq, ds.d) and queries.dtype == torch.float32 and queries.is_contiguous()
         self.queries = queries  # kept alive: the graph reads it
         self.ids = torch.empty((nq, k), dtype=torch.int32, device=queries.device)
         self.dists = torch.empty((nq, k), dtype=torch.float32, device=queries.device)
         o = Dataset._opts(mode, kprime, chunk_rows, group_queries, persistent, prepass_div, prepass_margin)
+        o.metas_on_device = int(self.any_batch)
         h = _P()
-        _check(lib().fbf_search_graph_create(ds.h, _P(_dev_ptr(queries)), _np_ptr(metas), nq, k, C.byref(o),
+        _check(lib().fbf_search_graph_create(ds.h, _P(_dev_ptr(queries)), mp, nq, k, C.byref(o),
                                              _P(_dev_ptr(self.ids)), _P(_dev_ptr(self.dists)), C.byref(h)),
                "fbf_search_graph_create")
         self.h = h
+
+    def plan_info(self) -> dict:
+        """Any-batch graphs, after a replay completed: the device plan's work
+        items per pass and the overflow flag."""
+        a, b, f = C.c_uint64(), C.c_uint64(), C.c_uint32()
+        _check(lib().fbf_search_graph_plan_info(self.h, C.byref(a), C.byref(b), C.byref(f)),
+               "fbf_search_graph_plan_info")
+        return {"items_prefix": a.value, "items_main": b.value, "overflow": bool(f.value)}
 
     def launch(self, stream=None):
         import torch
