@@ -205,6 +205,31 @@ def main() -> int:
         log(f"graph replays: {a.steps} in {dt:.3f}s: {graph_qps:.0f} QPS ({1e3 * dt / a.steps:.1f} ms/step), "
             f"ids equal to the eager search: {same}")
 
+    # Any-batch graph (metas in device memory): every replay routes the
+    # batch, plans it on the device and searches -- no host work at all.
+    any_qps = None
+    if mode in ("bf16", "fp8", "sq8", "sq8sym") and a.graph is not False:
+        kwg = {key: kw[key] for key in ("k", "mode", "kprime", "chunk_rows", "group_queries", "persistent",
+                                       "prepass_div", "prepass_margin")}
+        md = torch.from_numpy(np.ascontiguousarray(metas)).cuda()
+        g = pkg.fbf.SearchGraph(ds, qd, md, **kwg)
+        try:
+            g.launch()
+            torch.cuda.synchronize()
+            same = bool(np.array_equal(g.ids.cpu().numpy().view(np.uint32), ids))
+            info = g.plan_info()
+            t0 = time.perf_counter()
+            for i in range(a.steps):
+                g.launch()
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+        finally:
+            g.close()
+            del md
+        any_qps = nq * a.steps / dt
+        log(f"any-batch graph replays (device router + planner): {a.steps} in {dt:.3f}s: {any_qps:.0f} QPS "
+            f"({1e3 * dt / a.steps:.1f} ms/step), plan {info}, ids equal to the eager search: {same}")
+
     # The certified exact mode on the same dataset and batch (BF16 scan + FP32
     # rerank + certificate + FP32 fallback): exact results, timed like the
     # main line (1 warmup, then min(steps, 3) timed searches).
@@ -343,6 +368,7 @@ def main() -> int:
         "config": {"workload": cfg.name, "n": n, "d": 100, "nq": nq, "k": a.k, "mode": mode,
                    "kprime": kw["kprime"], "group_queries": kw["group_queries"], "chunk_rows": kw["chunk_rows"],
                    "autotuned": tuned, "per_batch_searches": per_batch, "graph_replay_qps": graph_qps,
+                   "any_batch_graph_replay_qps": any_qps,
                    "pairs_per_step": pairs, "work_items": stats["work_items"],
                    "certificate_fallbacks": fallbacks if mode == "exact_bf16" else None,
                    "certified_exact": exact_cert},

@@ -1408,13 +1408,14 @@ extern "C" fbf_status fbf_search(fbf_dataset *ds, const float *queries_dev, cons
 // ---------------------------------------------------------------- captured searches
 // Buffers of a plan built on the device (any-batch graphs).
 struct DPlanBufs {
-    DevBuf ranges, rkeys, rvals, keys, skeys, vals, order, hist, hoff, gd, gnc, qnc, ioff_pre, ioff_main,
-        items_tmp_pre, items_tmp_main, items_pre, items_main, islot_pre, islot_main, bins, boff, iq_pre, iq_main,
-        n_items, xoff, status;
+    DevBuf ranges, rkeys, rvals, keys, skeys, vals, order, hist, hoff, part, poff, gd, gnc, qnc, ioff_pre,
+        ioff_main, items_tmp_pre, items_tmp_main, items_pre, items_main, islot_pre, islot_main, bins, bcost, boff,
+        iq_pre, iq_main, n_items, xoff, status;
     void release() {
-        for (DevBuf *b : {&ranges, &rkeys, &rvals, &keys, &skeys, &vals, &order, &hist, &hoff, &gd, &gnc, &qnc,
-                          &ioff_pre, &ioff_main, &items_tmp_pre, &items_tmp_main, &items_pre, &items_main, &islot_pre,
-                          &islot_main, &bins, &boff, &iq_pre, &iq_main, &n_items, &xoff, &status})
+        for (DevBuf *b : {&ranges, &rkeys, &rvals, &keys, &skeys, &vals, &order, &hist, &hoff, &part, &poff, &gd,
+                          &gnc, &qnc, &ioff_pre, &ioff_main, &items_tmp_pre, &items_tmp_main, &items_pre,
+                          &items_main, &islot_pre, &islot_main, &bins, &bcost, &boff, &iq_pre, &iq_main, &n_items,
+                          &xoff, &status})
             b->release();
     }
 };
@@ -1472,7 +1473,11 @@ static fbf_status capture_any_batch(fbf_dataset *ds, fbf_search_graph *G, const 
     const bool seeded = pdiv > 1;
     const int B = K <= 160 ? 256 : K <= 300 ? 512 : 1024;
     const int B_pre = !seeded ? B : std::min(B, kpre <= 64 ? 128 : kpre <= 160 ? 256 : kpre <= 300 ? 512 : 1024);
-    const int nw = gmax <= kMfmaWaveQueries ? 1 : gmax <= 2 * kMfmaWaveQueries ? 2 : 4;
+    // Prefix items carry at most 128 queries (two waves per tile), as in the
+    // host plan: prefix lists start empty and compact often.
+    const uint32_t gpre = std::min<uint32_t>(gmax, 128u);
+    auto waves = [](uint32_t g) { return g <= kMfmaWaveQueries ? 1 : g <= 2 * kMfmaWaveQueries ? 2 : 4; };
+    const int nw = waves(gmax), nw_pre = waves(gpre);
     // Worst case: every query its own group over a whole layout.
     const uint64_t nchunk = (ds->n + kRowTile + chunk - 1) / chunk + 1;
     const uint64_t max_items = nq * nchunk;
@@ -1492,16 +1497,21 @@ static fbf_status capture_any_batch(fbf_dataset *ds, fbf_search_graph *G, const 
     for (DevBuf *b : {&D.vals, &D.order}) HIP_TRY(b->reserve(nq * sizeof(uint32_t)));
     HIP_TRY(D.hist.reserve(256 * nblk * sizeof(uint32_t)));
     HIP_TRY(D.hoff.reserve((256 * nblk + 1) * sizeof(uint64_t)));
-    HIP_TRY(D.gd.reserve(ngslot * 8 * sizeof(uint32_t)));
-    HIP_TRY(D.gnc.reserve(ngslot * 2 * sizeof(uint32_t)));
+    const uint64_t nparts = dplan_scan_parts(nq, gmax, ds->n);
+    HIP_TRY(D.part.reserve(nparts * sizeof(uint64_t)));
+    HIP_TRY(D.poff.reserve((nparts + 1) * sizeof(uint64_t)));
+    HIP_TRY(D.gd.reserve(ngslot * kGdStride * sizeof(uint32_t)));
+    HIP_TRY(D.gnc.reserve(ngslot * kGncStride * sizeof(uint32_t)));
     HIP_TRY(D.qnc.reserve(nq * 2 * sizeof(uint32_t)));
     HIP_TRY(D.ioff_pre.reserve((ngslot + 1) * sizeof(uint64_t)));
     HIP_TRY(D.ioff_main.reserve((ngslot + 1) * sizeof(uint64_t)));
     for (DevBuf *b : {&D.items_tmp_pre, &D.items_tmp_main, &D.items_pre, &D.items_main})
         HIP_TRY(b->reserve(max_items * sizeof(WorkItem)));
     for (DevBuf *b : {&D.islot_pre, &D.islot_main}) HIP_TRY(b->reserve(max_items * sizeof(uint32_t)));
-    HIP_TRY(D.bins.reserve(2ull * nbins * sizeof(uint32_t)));
-    HIP_TRY(D.boff.reserve(2ull * (nbins + 1) * sizeof(uint64_t)));
+    const uint32_t nrb = dplan_row_bins(ds->n);
+    HIP_TRY(D.bins.reserve(2ull * nrb * sizeof(uint32_t)));
+    HIP_TRY(D.boff.reserve(2ull * (nrb + 1) * sizeof(uint64_t)));
+    HIP_TRY(D.bcost.reserve(2ull * nbins * sizeof(uint64_t)));
     HIP_TRY(D.iq_pre.reserve(nq * sizeof(ItemQuery)));
     HIP_TRY(D.iq_main.reserve(nq * sizeof(ItemQuery)));
     HIP_TRY(D.n_items.reserve(2 * sizeof(uint64_t)));
@@ -1523,6 +1533,7 @@ static fbf_status capture_any_batch(fbf_dataset *ds, fbf_search_graph *G, const 
     dp.nq = nq;
     dp.n_rows = ds->n;
     dp.gmax = gmax;
+    dp.gpre = gpre;
     dp.chunk = chunk;
     dp.pdiv = seeded ? pdiv : 1u;
     dp.min_prefix = (uint32_t)std::max<uint64_t>(kMinPrefixRows, 2ull * kpre);
@@ -1534,6 +1545,8 @@ static fbf_status capture_any_batch(fbf_dataset *ds, fbf_search_graph *G, const 
     dp.order = D.order.as<uint32_t>();
     dp.hist = D.hist.as<uint32_t>();
     dp.hoff = D.hoff.as<uint64_t>();
+    dp.part = D.part.as<uint64_t>();
+    dp.poff = D.poff.as<uint64_t>();
     dp.gd = D.gd.as<uint32_t>();
     dp.gnc = D.gnc.as<uint32_t>();
     dp.qnc = D.qnc.as<uint32_t>();
@@ -1548,6 +1561,7 @@ static fbf_status capture_any_batch(fbf_dataset *ds, fbf_search_graph *G, const 
     dp.islot_pre = D.islot_pre.as<uint32_t>();
     dp.islot_main = D.islot_main.as<uint32_t>();
     dp.bins = D.bins.as<uint32_t>();
+    dp.bcost = D.bcost.as<unsigned long long>();
     dp.boff = D.boff.as<uint64_t>();
     dp.iq_pre = D.iq_pre.as<ItemQuery>();
     dp.iq_main = D.iq_main.as<ItemQuery>();
@@ -1558,7 +1572,10 @@ static fbf_status capture_any_batch(fbf_dataset *ds, fbf_search_graph *G, const 
 
     const int variant = o.mode == FBF_MODE_BF16 ? 0 : o.mode == FBF_MODE_FP8 ? 1 : o.mode == FBF_MODE_SQ8 ? 2 : 3;
     warm_scan_mfma(variant, B, nw);
-    if (seeded) warm_scan_mfma(variant, B_pre, nw);
+    if (seeded) {
+        warm_scan_mfma(variant, B_pre, nw_pre);
+        warm_scan_mfma(variant, B, nw_pre);
+    }
     HIP_TRY(hipMemsetAsync(D.status.p, 0, 4 * sizeof(uint32_t), st));  // before the capture: no memset node
     HIP_TRY(hipStreamSynchronize(st));
     HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -1599,7 +1616,7 @@ static fbf_status capture_any_batch(fbf_dataset *ds, fbf_search_graph *G, const 
             a.qmask = mask;
             a.nmask = mask ? mask + nq : nullptr;
             HIP_TRY(launch_fill_u32(queue, 0u, 8 * 16, st));
-            HIP_TRY(launch_scan_mfma(variant, a, cap_b, nw, st));
+            HIP_TRY(launch_scan_mfma(variant, a, cap_b, pass ? nw : nw_pre, st));
             return FBF_OK;
         };
         MergeArgs m{};

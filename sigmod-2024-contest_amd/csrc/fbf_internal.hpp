@@ -257,33 +257,41 @@ hipError_t launch_scan_mfma(int variant, const ScanMfmaArgs &a, int B, int nw, h
 // memory (fbf_dplan.hip).  All pointers are device buffers of the stated
 // capacity; n_items[0] / [1] receive the item counts of the two passes,
 // status = {items of the prefix pass, items of the main pass, overflow}.
+constexpr uint32_t kGdStride = 12;   // u32 per group descriptor of the device planner
+constexpr uint32_t kGncStride = 4;   // u32 per group's item counts (prefix halves, main)
 struct DPlan {
     const uint32_t *ranges;  // [nq][4] {layout, s, e, -}
     uint64_t nq;
     uint64_t n_rows;         // rows of each layout
-    uint32_t gmax, chunk, pdiv, min_prefix;
+    uint32_t gmax, gpre, chunk, pdiv, min_prefix;  // gpre: members per prefix item (gmax <= 2 gpre)
     uint64_t max_items, max_slots;
     uint64_t *keys, *skeys;        // [nq] (ping-pong buffers of the query sort)
     uint32_t *vals, *order;        // [nq]
     uint32_t *hist;                // [256 * ceil(nq / 256)]
     uint64_t *hoff;                // [256 * ceil(nq / 256) + 1]
-    uint32_t *gd;                  // [ceil(nq / gmax) * gmax][8]
-    uint32_t *gnc;                 // [ceil(nq / gmax) * gmax][2]
+    uint64_t *part, *poff;         // [dplan_scan_parts], [dplan_scan_parts + 1]
+    uint32_t *gd;                  // [ceil(nq / gmax) * gmax][kGdStride]
+    uint32_t *gnc;                 // [ceil(nq / gmax) * gmax][kGncStride]
     uint32_t *qnc;                 // [nq][2]
     uint64_t *ioff_pre, *ioff_main;  // [ceil(nq / gmax) * gmax + 1]
     uint64_t *slot_off;            // [nq + 1]
     uint64_t *pre_end;             // [nq]
     WorkItem *items_tmp_pre, *items_tmp_main, *items_pre, *items_main;  // [max_items] each
     uint32_t *islot_pre, *islot_main;  // [max_items] each: an item's place in its row bin
-    uint32_t *bins;                // [2][dplan_bins(n_rows, chunk)]: items per row bin
-    uint64_t *boff;                // [2][dplan_bins(n_rows, chunk) + 1]
+    uint32_t *bins;                // [2][dplan_row_bins(n_rows)]: items per first row
+    uint64_t *boff;                // [2][dplan_row_bins(n_rows) + 1]
+    unsigned long long *bcost;     // [2][dplan_bins(n_rows, chunk)]: rows x queries per chunk bin
     ItemQuery *iq_pre, *iq_main;   // [nq] each (indexed by sorted position)
     uint64_t *n_items;             // [2]
     uint32_t *xoff_pre, *xoff_main;  // [9] each
-    uint32_t *status;              // [4]
+    uint32_t *status;              // [4]: items per pass, overflow, list slots
 };
-// Row bins of the item order: both layouts' rows cut at multiples of chunk.
+// Bins of the item order (first rows: multiples of kRowTile) and of the XCD
+// split (multiples of chunk), both layouts.
+inline uint32_t dplan_row_bins(uint64_t n_rows) { return (uint32_t)(2 * (n_rows / kRowTile + 1)); }
 inline uint32_t dplan_bins(uint64_t n_rows, uint32_t chunk) { return (uint32_t)(2 * (n_rows / chunk + 1)); }
+// Tile sums the planner's scans hold at most.
+uint64_t dplan_scan_parts(uint64_t nq, uint32_t gmax, uint64_t n_rows);
 hipError_t launch_dplan(const DPlan &dp, hipStream_t st);
 // B-operand fragments of a batch's queries for the MFMA scans, kQFragU4
 // uint4 per query (variant 3 reads the queries' symmetric int8 codes).
