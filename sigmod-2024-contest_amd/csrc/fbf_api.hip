@@ -134,7 +134,7 @@ struct Scratch {
     hipEvent_t ev_done = nullptr;
     bool used = false;
     void release() {
-        for (DevBuf *b : {&d_items, &d_iq, &d_slot_off, &d_lists, &d_counts, &d_queue, &d_q, &d_out_ids,
+        for (DevBuf *b : {&d_items, &d_iq, &d_slot_off, &d_lists, &d_counts, &d_queue, &d_q, &d_qf, &d_out_ids,
                           &d_out_d, &d_gtau, &d_tau0, &d_flags, &d_pre_end, &d_cert, &d_sub_idx, &d_sub_q,
                           &d_sub_ids, &d_sub_d, &d_ti_sl, &d_ti_lists, &d_ti_counts, &d_ti_off, &d_ti_ext})
             b->release();
