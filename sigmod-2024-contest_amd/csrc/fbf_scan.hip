@@ -226,16 +226,43 @@ __global__ __launch_bounds__(64) void merge_kernel(MergeArgs a) {
     uint32_t cnt = 0;
     uint64_t tau = kKeyMax;
     const uint64_t s0 = a.slot_off[q], s1 = a.pass == 3 ? a.pre_end[q] : a.slot_off[q + 1];
+    // 64 slots at a time: lane j reads the count word of slot sb + j, a wave
+    // prefix sum numbers the keys of those slots, and each batch of 64 keys
+    // is gathered with one load per lane (lane l finds its slot by a binary
+    // search over the prefix sums), so no load waits for another.
 #pragma unroll 1
-    for (uint64_t si = s0; si < s1; ++si) {
-        const uint64_t slot = si;
-        const uint32_t w = a.counts[slot];
-        const uint32_t c0 = list_count_c0(w), c = c0 + list_count_c1(w), h = list_count_h(w);
-        const uint64_t *L = a.lists + slot * (uint64_t)a.list_cap;
+    for (uint64_t sb = s0; sb < s1; sb += 64) {
+        const uint64_t si = sb + lane;
+        uint32_t c0 = 0, c = 0, h = 0;
+        if (si < s1) {
+            const uint32_t w = a.counts[si];
+            c0 = list_count_c0(w);
+            c = c0 + list_count_c1(w);
+            h = list_count_h(w);
+        }
+        uint32_t incl = c;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = (uint32_t)__shfl_up((int)incl, o);
+            if (lane >= (uint32_t)o) incl += y;
+        }
+        const uint32_t total = (uint32_t)__shfl((int)incl, 63);
 #pragma unroll 1
-        for (uint32_t i = 0; i < c; i += 64) {
-            const uint32_t ii = i + lane;
-            const uint64_t key = ii < c ? L[ii < c0 ? ii : h + (ii - c0)] : kKeyMax;
+        for (uint32_t g0 = 0; g0 < total; g0 += 64) {
+            const uint32_t g = g0 + lane < total ? g0 + lane : total - 1;
+            uint32_t lo = 0, hi = 63;  // the first slot whose inclusive count exceeds g
+#pragma unroll
+            for (int it = 0; it < 6; ++it) {
+                const uint32_t mid = (lo + hi) >> 1;
+                const uint32_t v = (uint32_t)__shfl((int)incl, (int)mid);
+                if (v > g) hi = mid;
+                else lo = mid + 1;
+            }
+            const uint32_t e = (uint32_t)__shfl((int)(incl - c), (int)lo);
+            const uint32_t cj0 = (uint32_t)__shfl((int)c0, (int)lo), hj = (uint32_t)__shfl((int)h, (int)lo);
+            const uint32_t ii = g - e;
+            const uint64_t *L = a.lists + (sb + lo) * (uint64_t)a.list_cap;
+            const uint64_t key = g0 + lane < total ? L[ii < cj0 ? ii : hj + (ii - cj0)] : kKeyMax;
             offer_lds<B>(list, cnt, tau, a.kprime, key < tau, key);
         }
     }
