@@ -293,7 +293,8 @@ __global__ __launch_bounds__(64) void merge_kernel(MergeArgs a) {
 
     if (a.rerank) {
         const float *__restrict__ qv = a.queries + q * kDim;
-        for (uint32_t i = lane; i < (uint32_t)B; i += 64) {
+        const uint32_t nsort = pow2_at_least(cnt);
+        for (uint32_t i = lane; i < nsort; i += 64) {
             uint64_t nk = kKeyMax;
             if (i < cnt) {
                 const uint32_t low = (uint32_t)list[i];
@@ -304,7 +305,7 @@ __global__ __launch_bounds__(64) void merge_kernel(MergeArgs a) {
             list[i] = nk;
         }
         __syncthreads();
-        wave_sort_lds<B>(list);
+        wave_sort_lds_n(list, nsort);
         if (a.cert) {
             // Rows outside the candidates have BF16 scores at or above the
             // k'-th one, ts (keys are unique, so "outside" means "key above
@@ -395,13 +396,14 @@ hipError_t launch_scatter_results(const uint32_t *ids, const float *d, const uin
 hipError_t launch_merge(const MergeArgs &a, hipStream_t st) {
     if (a.nq == 0) return hipSuccess;
     const uint32_t need = a.kprime > a.k ? a.kprime : a.k;
-    const int B = list_capacity((int)need);
-    // The LDS list keeps need keys plus one 64-key batch of offers (offer_lds).
+    // The LDS list keeps need keys plus batches of 64 offers (offer_lds):
+    // room for several batches between compactions (each compaction sorts
+    // the list), 8 KiB per query up to 416 kept keys, 16 KiB beyond.
+    const int B = need <= 416 ? 1024 : 2048;
     if ((int)need + 64 > B) return hipErrorInvalidValue;
     switch (B) {
-    case 256: hipLaunchKernelGGL(merge_kernel<256>, dim3((uint32_t)a.nq), dim3(64), 0, st, a); break;
-    case 512: hipLaunchKernelGGL(merge_kernel<512>, dim3((uint32_t)a.nq), dim3(64), 0, st, a); break;
     case 1024: hipLaunchKernelGGL(merge_kernel<1024>, dim3((uint32_t)a.nq), dim3(64), 0, st, a); break;
+    case 2048: hipLaunchKernelGGL(merge_kernel<2048>, dim3((uint32_t)a.nq), dim3(64), 0, st, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

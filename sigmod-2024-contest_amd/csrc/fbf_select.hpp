@@ -81,6 +81,41 @@ __device__ inline void wave_sort_lds(uint64_t *s) {
     }
 }
 
+// Ascending bitonic sort of the first n keys in LDS (n a power of two, at
+// least 2) by one wave.
+template <class Sync = BlockSync>
+__device__ inline void wave_sort_lds_n(uint64_t *s, uint32_t n) {
+    const uint32_t lane = threadIdx.x & 63;
+#pragma unroll 1
+    for (uint32_t size = 2; size <= n; size <<= 1) {
+#pragma unroll 1
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+#pragma unroll 1
+            for (uint32_t p0 = 0; p0 < n / 2; p0 += 64) {
+                const uint32_t p = p0 + lane;
+                if (p < n / 2) {
+                    const uint32_t i = 2 * p - (p & (stride - 1));
+                    const uint32_t j = i + stride;
+                    const bool up = (i & size) == 0;
+                    const uint64_t x = s[i], y = s[j];
+                    if ((x > y) == up) {
+                        s[i] = y;
+                        s[j] = x;
+                    }
+                }
+            }
+            Sync::sync();
+        }
+    }
+}
+
+// The smallest power of two at or above max(c, 2).
+__device__ inline uint32_t pow2_at_least(uint32_t c) {
+    uint32_t n = 2;
+    while (n < c) n <<= 1;
+    return n;
+}
+
 // A query's candidate list in global memory (list) with its count and
 // threshold.  The LDS buffer sbuf (B keys) is scratch for compaction.
 template <int B, class Sync = BlockSync>
@@ -106,18 +141,16 @@ __device__ inline void compact_global(uint64_t *list, uint32_t &cnt, uint64_t &t
     Sync::sync();
 }
 
-// The same with the list itself in LDS (merge kernels).
+// The same with the list itself in LDS (merge kernels); only the smallest
+// power of two of keys holding the list is sorted.
 template <int B>
 __device__ inline void compact_lds(uint64_t *list, uint32_t &cnt, uint64_t &tau, uint32_t K) {
     const uint32_t lane = threadIdx.x;
+    const uint32_t n = pow2_at_least(cnt);
     __syncthreads();
-#pragma unroll
-    for (int i0 = 0; i0 < B; i0 += 64) {
-        const uint32_t i = i0 + lane;
-        if (i >= cnt) list[i] = kKeyMax;
-    }
+    for (uint32_t i = cnt + lane; i < n; i += 64) list[i] = kKeyMax;
     __syncthreads();
-    wave_sort_lds<B>(list);
+    wave_sort_lds_n(list, n);
     const uint32_t keep = cnt < K ? cnt : K;
     tau = (keep == K) ? uniform_u64(list[K - 1]) : kKeyMax;
     cnt = keep;
