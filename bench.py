@@ -226,9 +226,10 @@ def main() -> int:
         finally:
             g.close()
             del md
-        any_qps = nq * a.steps / dt
-        log(f"any-batch graph replays (device router + planner): {a.steps} in {dt:.3f}s: {any_qps:.0f} QPS "
-            f"({1e3 * dt / a.steps:.1f} ms/step), plan {info}, ids equal to the eager search: {same}")
+        any_qps = nq * a.steps / dt if not info["overflow"] else None
+        log(f"any-batch graph replays (device router + planner): {a.steps} in {dt:.3f}s: "
+            + (f"{any_qps:.0f} QPS ({1e3 * dt / a.steps:.1f} ms/step)" if any_qps else "plan over the graph's capacity")
+            + f", plan {info}, ids equal to the eager search: {same}")
 
     # The certified exact mode on the same dataset and batch (BF16 scan + FP32
     # rerank + certificate + FP32 fallback): exact results, timed like the

@@ -171,7 +171,8 @@ fbf_status fbf_search_host(fbf_dataset *ds, const float *queries, const float *m
  * queries and metas (the plans differ; the kept keys do not).  Unknown filter
  * types and empty ranges give padded outputs (id 0xFFFFFFFF, distance
  * +inf).  Buffers are sized at creation for the worst plan of nq queries,
- * the candidate lists at most 32 GB; a replay whose plan needs more list
+ * the candidate lists at most 32 GB or 3/4 of the device memory free at
+ * creation, whichever is more; a replay whose plan needs more list
  * slots than that returns padded outputs for every query and sets the
  * overflow flag of fbf_search_graph_plan_info (re-run such a batch with
  * fbf_search or with larger chunk_rows). */

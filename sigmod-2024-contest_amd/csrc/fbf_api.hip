@@ -1439,7 +1439,8 @@ static fbf_status kprime_of(const fbf_search_opts &o, uint32_t k, uint32_t &K) {
 }
 
 // Candidate-list memory of a graph that plans on the device: room for every
-// query's lists at the worst plan, at most this many bytes.
+// query's lists at the worst plan, at most this many bytes or three quarters
+// of the device memory free at creation, whichever is more.
 constexpr uint64_t kAnyBatchListBytes = 32ull << 30;
 
 // An any-batch graph: the router kernel, the device planner (fbf_dplan.hip)
@@ -1481,15 +1482,18 @@ static fbf_status capture_any_batch(fbf_dataset *ds, fbf_search_graph *G, const 
     // Worst case: every query its own group over a whole layout.
     const uint64_t nchunk = (ds->n + kRowTile + chunk - 1) / chunk + 1;
     const uint64_t max_items = nq * nchunk;
+    HIP_TRY(hipSetDevice(ds->device));
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    const uint64_t list_cap = std::max<uint64_t>(kAnyBatchListBytes, (uint64_t)free_b / 4 * 3);
     const uint64_t max_slots =
-        std::min<uint64_t>({2 * nq * nchunk, kAnyBatchListBytes / ((uint64_t)B * sizeof(uint64_t)), 0xFFFFFFF0ull});
+        std::min<uint64_t>({2 * nq * nchunk, list_cap / ((uint64_t)B * sizeof(uint64_t)), 0xFFFFFFF0ull});
     if (max_items >= 0xFFFFFFF0ull) return fail(FBF_ERR_INVALID_ARG, "batch too large for a graph that plans on the device");
     const uint64_t nwin = (nq + gmax - 1) / gmax, ngslot = nwin * gmax;
     const uint64_t nblk = (nq + 255) / 256;
     const uint32_t nbins = dplan_bins(ds->n, chunk);
     Scratch &S = G->scr;
     DPlanBufs &D = G->dp;
-    HIP_TRY(hipSetDevice(ds->device));
     HIP_TRY(D.ranges.reserve(nq * 4 * sizeof(uint32_t)));
     HIP_TRY(D.rkeys.reserve(nq * 4 * sizeof(uint64_t)));
     HIP_TRY(D.rvals.reserve(nq * 2 * sizeof(uint32_t)));
