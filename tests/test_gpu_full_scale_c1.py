@@ -15,6 +15,8 @@ BF16 and FP8 MFMA with FP32 rerank, symmetric int8) on the whole batch.
 * symmetric int8 at k' = 140 is the router's brute-force route: equal to the
   oracle's restatement (orc_router_bf) on the sample, ids and distances.
 * fbf_search's sub-batch split (max_batch) returns the same as one search.
+* the any-batch graph (router and planner on the device) equals eager
+  searches on the batch, a permutation of it and a batch of other filters.
 """
 import sys
 
@@ -209,3 +211,55 @@ def test_c1_device_router(orc, c1):
         torch.cuda.synchronize()
         assert np.array_equal(ids.cpu().numpy().view(np.uint32), a[0]), mode
         assert np.array_equal(d.cpu().numpy(), a[1]), mode
+
+
+def test_c1_any_batch_graph(pkg, c1):
+    """The any-batch graph (router, device planner and search captured
+    together) on the whole batch: a replay equals the eager search; after the
+    metas and vectors are refilled with a permutation of the batch (every
+    group and window of the device plan changes) the replay equals the eager
+    results permuted the same way; a third batch of other filters (every
+    query given a time window) equals its own eager search."""
+    import torch
+    ds = c1["ds"]
+    m = c1["metas"]
+    nq = m.shape[0]
+    _say("case: N=10M nq=100k any-batch graph (bf16) vs eager, a permuted batch and a batch of other filters")
+    ref = _search(c1, mode="bf16")
+    qd = c1["qd"].clone()
+    md = torch.from_numpy(np.ascontiguousarray(m)).cuda()
+    g = pkg.fbf.SearchGraph(ds, qd, md, k=K, mode="bf16")
+    try:
+        ids, d = g.launch()
+        torch.cuda.synchronize()
+        info = g.plan_info()
+        _say(f"  plan {info}")
+        assert not info["overflow"]
+        assert np.array_equal(ids.cpu().numpy().view(np.uint32), ref[0])
+        assert np.array_equal(d.cpu().numpy(), ref[1])
+        perm = np.random.default_rng(5).permutation(nq)
+        pt = torch.from_numpy(perm).cuda()
+        qd.copy_(c1["qd"][pt])
+        md.copy_(torch.from_numpy(np.ascontiguousarray(m[perm])).cuda())
+        ids, d = g.launch()
+        torch.cuda.synchronize()
+        assert not g.plan_info()["overflow"]
+        assert np.array_equal(ids.cpu().numpy().view(np.uint32), ref[0][perm])
+        assert np.array_equal(d.cpu().numpy(), ref[1][perm])
+        m3 = m.copy()
+        rng = np.random.default_rng(9)
+        lr = np.sort(rng.random((nq, 2), dtype=np.float32), axis=1)
+        m3[:, 2:] = lr
+        m3[m3[:, 0] == 0, 0] = 2
+        m3[m3[:, 0] == 1, 0] = 3
+        qd.copy_(c1["qd"])
+        md.copy_(torch.from_numpy(m3).cuda())
+        ids, d = g.launch()
+        torch.cuda.synchronize()
+        assert not g.plan_info()["overflow"]
+        e_ids, e_d = ds.search(c1["qd"], m3, k=K, mode="bf16")
+        torch.cuda.synchronize()
+        assert np.array_equal(ids.cpu().numpy(), e_ids.cpu().numpy())
+        assert np.array_equal(d.cpu().numpy(), e_d.cpu().numpy())
+    finally:
+        g.close()
