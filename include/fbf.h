@@ -258,7 +258,10 @@ fbf_status fbf_unregister_codes(const void *host_base);
  * to codes_in in ascending (distance, position) order -- the same SET as the
  * reference's heap whenever the k-th distance is not tied.  dim, alpha and
  * nqueries are accepted for signature parity (the reference ignores them).
- * k > npoints pads with -1. */
+ * k > npoints pads with -1.  Calls on registered arrays from several threads
+ * at once are combined: a waiting caller runs every queued call of its GPU as
+ * one launch (each call still gets exactly its own results), so the router's
+ * OpenMP loop keeps the GPU busy without changing its call pattern. */
 fbf_status fbf_bruteforce(size_t dim, size_t k, int *closest_points, size_t npoints,
                           const char *codes_in, float alpha, size_t nqueries,
                           const int8_t *queries_in);

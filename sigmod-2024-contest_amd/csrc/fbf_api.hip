@@ -12,6 +12,7 @@
 #include <thread>
 #include <cmath>
 #include <cstdio>
+#include <condition_variable>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -1969,6 +1970,22 @@ extern "C" fbf_status fbf_unregister_codes(const void *host_base) {
     return FBF_OK;
 }
 
+// The registered array holding [p, p + bytes): its device base and size.
+static bool resolve_codes_array(const void *p, uint64_t bytes, const int8_t *&arr, uint64_t &arr_bytes,
+                                uint64_t &offset) {
+    std::lock_guard<std::mutex> lk(g_reg_mu);
+    const uintptr_t a = (uintptr_t)p;
+    auto it = g_reg.upper_bound(a);
+    if (it == g_reg.begin()) return false;
+    --it;
+    const uintptr_t base = it->first;
+    if (a < base || a + bytes > base + it->second.bytes) return false;
+    arr = it->second.dev;
+    arr_bytes = it->second.bytes;
+    offset = a - base;
+    return true;
+}
+
 // Device address of [p, p + bytes) when it lies inside one registered array.
 static const int8_t *resolve_codes(const void *p, uint64_t bytes) {
     std::lock_guard<std::mutex> lk(g_reg_mu);
@@ -1996,6 +2013,9 @@ struct ThreadEngine {
     DevBuf tmp_codes;
     HostPinned h_out;
     uint32_t *done = nullptr;  // arrival counter of sym_one_kernel (zero between calls)
+    DevBuf d_reqs, d_done;     // combined calls: requests; arrival counters (zero between calls), then
+                               // shared thresholds (all ones between calls)
+    HostPinned h_reqs;
 };
 static std::mutex g_eng_mu;
 static std::vector<ThreadEngine *> *g_eng_free = new std::vector<ThreadEngine *>();  // never destroyed
@@ -2205,6 +2225,152 @@ static fbf_status sym_topk(const int8_t *codes_dev, uint64_t n_rows, const uint6
     return FBF_OK;
 }
 
+// Combining of concurrent per-call searches (the router's OpenMP threads call
+// bruteforce() once per query, hybrid_graph.cpp:168-211).  A call on a
+// registered code array queues its request; a waiting caller that finds
+// fewer than kBfLeaders batches in flight takes every queued request of its
+// device and runs them on its own engine as batches (one per code array and
+// k: one plan, one scan launch, one merge, one copy back), then hands each
+// request its results.  A lone request runs as today's single launch.  Each
+// request's results are those of its own call: the batch computes every
+// (slice, query) pair independently.
+struct BfReq {
+    const int8_t *arr = nullptr;    // device base of the registered code array
+    uint64_t arr_rows = 0;          // its whole rows
+    uint64_t start = 0;             // the slice: rows [start, start + n) of the array
+    uint64_t n = 0;
+    const int8_t *q = nullptr;      // the query's 112 code bytes (host)
+    uint32_t k = 0;
+    int32_t *pos = nullptr, *dist = nullptr;  // host outputs [k]
+    fbf_status status = FBF_OK;
+    std::string err;
+    bool done = false;
+};
+constexpr int kBfLeaders = 2;       // batches in flight per device
+constexpr size_t kBfMaxBatch = 512;  // requests per batch
+struct BfCombiner {
+    std::mutex mu;
+    std::condition_variable cv;
+    std::vector<BfReq *> queue;
+    int leaders = 0;
+};
+static BfCombiner *g_bf = new BfCombiner[64];  // per device, never destroyed
+
+// Run one batch of requests of one device on the calling thread's engine:
+// every request a one-slice search of its own, all in one launch
+// (sym_multi_kernel), at the largest k of the batch (a request's own top-k is
+// the head of that sorted list); one upload of the requests, one copy back.
+static fbf_status bf_run_batch_(std::vector<BfReq *> &batch) {
+    ThreadEngine *Ep = nullptr;
+    fbf_status s = thread_engine(Ep);
+    if (s != FBF_OK) return s;
+    ThreadEngine &E = *Ep;
+    const hipStream_t st = E.st;
+    const uint32_t nreq = (uint32_t)batch.size();
+    uint32_t K = 0;
+    for (const BfReq *r : batch) K = std::max(K, r->k);
+    const int B = list_capacity((int)K);
+    HIP_TRY(E.h_reqs.reserve(nreq * sizeof(SymReq)));
+    SymReq *hr = static_cast<SymReq *>(E.h_reqs.p);
+    uint64_t slots = 0;
+    uint32_t gx = 0;
+    for (uint32_t i = 0; i < nreq; ++i) {
+        const BfReq &r = *batch[i];
+        // At least 2048 rows per workgroup and at most 64 workgroups: each
+        // keeps its own top k and the last one merges them all, so fewer,
+        // longer workgroups (their tiles prefetched) cost less in all.  On
+        // slices like the router's (1 Ki to 800 Ki rows) from 16 threads:
+        // 26.5 k calls/s, against 17.9 k at 1 Ki rows and up to 256.
+        uint64_t chunk = std::max<uint64_t>(2048, (r.n + 63) / 64);
+        chunk = (chunk + kRowTile - 1) / kRowTile * kRowTile;
+        const uint32_t nblk = (uint32_t)((r.n + chunk - 1) / chunk);
+        SymReq q{};
+        q.codes = r.arr + r.start * 112;
+        q.n = (uint32_t)r.n;
+        q.chunk = (uint32_t)chunk;
+        q.nblk = nblk;
+        q.list0 = (uint32_t)slots;
+        std::memcpy(q.q, r.q, 112);
+        hr[i] = q;
+        slots += nblk;
+        gx = std::max(gx, nblk);
+    }
+    Scratch &S = E.scr;
+    HIP_TRY(S.d_lists.reserve(slots * (uint64_t)B * sizeof(uint64_t)));
+    HIP_TRY(S.d_counts.reserve(slots * sizeof(uint32_t)));
+    HIP_TRY(S.d_out_ids.reserve((size_t)nreq * 2 * K * sizeof(uint32_t)));
+    HIP_TRY(E.d_reqs.reserve(nreq * sizeof(SymReq)));
+    // [done: kBfMaxBatch u32][gtau: kBfMaxBatch u64], set once.
+    const size_t done_bytes = kBfMaxBatch * (sizeof(uint32_t) + sizeof(uint64_t));
+    if (E.d_done.bytes < done_bytes) {
+        HIP_TRY(E.d_done.reserve(done_bytes));
+        HIP_TRY(hipMemsetAsync(E.d_done.p, 0, kBfMaxBatch * sizeof(uint32_t), st));
+        HIP_TRY(hipMemsetAsync(E.d_done.as<uint32_t>() + kBfMaxBatch, 0xFF, kBfMaxBatch * sizeof(uint64_t), st));
+    }
+    if (nreq > kBfMaxBatch) return fail(FBF_ERR_INVALID_ARG, "too many combined requests");
+    HIP_TRY(hipMemcpyAsync(E.d_reqs.p, hr, nreq * sizeof(SymReq), hipMemcpyHostToDevice, st));
+    SymMultiArgs a{};
+    a.reqs = E.d_reqs.as<SymReq>();
+    a.nreq = nreq;
+    a.K = K;
+    a.lists = S.d_lists.as<uint64_t>();
+    a.counts = S.d_counts.as<uint32_t>();
+    a.done = E.d_done.as<uint32_t>();
+    a.gtau = reinterpret_cast<uint64_t *>(E.d_done.as<uint32_t>() + kBfMaxBatch);
+    a.out = S.d_out_ids.as<uint32_t>();
+    HIP_TRY(launch_sym_multi(a, gx, B, st));
+    HIP_TRY(E.h_out.reserve((size_t)nreq * 2 * K * sizeof(uint32_t)));
+    uint32_t *ho = static_cast<uint32_t *>(E.h_out.p);
+    HIP_TRY(hipMemcpyAsync(ho, a.out, (size_t)nreq * 2 * K * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (uint32_t i = 0; i < nreq; ++i) {
+        BfReq &r = *batch[i];
+        const uint32_t *o = ho + (size_t)i * 2 * K;
+        for (uint32_t j = 0; j < r.k; ++j) {
+            const bool pad = o[j] == kPadId;
+            r.pos[j] = pad ? -1 : (int32_t)o[j];
+            r.dist[j] = pad ? INT32_MAX : (int32_t)o[K + j];
+        }
+    }
+    return FBF_OK;
+}
+
+static void bf_run_batch(std::vector<BfReq *> &batch) {
+    const fbf_status s = bf_run_batch_(batch);
+    const std::string msg = s == FBF_OK ? std::string() : g_err;
+    for (BfReq *r : batch) {
+        r->status = s;
+        r->err = msg;
+    }
+}
+
+// Queue a request and wait for its results (running batches meanwhile when
+// fewer than kBfLeaders are in flight).
+static fbf_status bf_combined(BfReq &r, int dev) {
+    BfCombiner &C = g_bf[dev];
+    std::unique_lock<std::mutex> lk(C.mu);
+    C.queue.push_back(&r);
+    while (!r.done) {
+        if (C.leaders < kBfLeaders && !C.queue.empty()) {
+            ++C.leaders;
+            std::vector<BfReq *> batch;
+            const size_t m = std::min(C.queue.size(), kBfMaxBatch);
+            batch.assign(C.queue.begin(), C.queue.begin() + (long)m);
+            C.queue.erase(C.queue.begin(), C.queue.begin() + (long)m);
+            lk.unlock();
+            bf_run_batch(batch);
+            lk.lock();
+            for (BfReq *b : batch) b->done = true;
+            --C.leaders;
+            C.cv.notify_all();
+        } else {
+            C.cv.wait(lk);
+        }
+    }
+    if (r.status != FBF_OK) return fail(r.status, r.err);
+    return FBF_OK;
+}
+
 static fbf_status bruteforce_common(size_t k, int *closest_points, float *distances, size_t npoints,
                                     const char *codes_in, float alpha, const int8_t *queries_in) {
     if (k == 0) return FBF_OK;  // the reference writes nothing either
@@ -2217,7 +2383,30 @@ static fbf_status bruteforce_common(size_t k, int *closest_points, float *distan
     if (npoints > 0) {
         const uint64_t bytes = (uint64_t)npoints * 112;
         const int8_t *dev = resolve_codes(codes_in, bytes);
-        if (!dev) {
+        const int8_t *arr = nullptr;
+        uint64_t arr_bytes = 0, offset = 0;
+        if (dev && resolve_codes_array(codes_in, bytes, arr, arr_bytes, offset) && offset % 112 == 0) {
+            // A slice of whole rows of a registered array: combined with the
+            // other threads' calls.
+            int d = 0;
+            HIP_TRY(hipGetDevice(&d));
+            if (d < 0 || d >= 64) return fail(FBF_ERR_UNSUPPORTED, "device index beyond the combiner table");
+            BfReq r;
+            r.arr = arr;
+            r.arr_rows = arr_bytes / 112;
+            r.start = offset / 112;
+            r.n = npoints;
+            r.q = queries_in;
+            r.k = (uint32_t)k;
+            r.pos = pos.data();
+            r.dist = dist.data();
+            fbf_status s = bf_combined(r, d);
+            if (s != FBF_OK) return s;
+        } else if (dev) {
+            const uint64_t st0 = 0, cnt = npoints;
+            fbf_status s = sym_topk(dev, npoints, &st0, &cnt, queries_in, 1, (uint32_t)k, pos.data(), dist.data());
+            if (s != FBF_OK) return s;
+        } else {
             // Not registered: stage this slice (correct, but a copy per call).
             ThreadEngine *Ep = nullptr;
             fbf_status se = thread_engine(Ep);
@@ -2225,11 +2414,11 @@ static fbf_status bruteforce_common(size_t k, int *closest_points, float *distan
             ThreadEngine &E = *Ep;
             HIP_TRY(E.tmp_codes.reserve(bytes));
             HIP_TRY(hipMemcpyAsync(E.tmp_codes.p, codes_in, bytes, hipMemcpyHostToDevice, E.st));
-            dev = E.tmp_codes.as<int8_t>();
+            const uint64_t st0 = 0, cnt = npoints;
+            fbf_status s = sym_topk(E.tmp_codes.as<int8_t>(), npoints, &st0, &cnt, queries_in, 1, (uint32_t)k,
+                                    pos.data(), dist.data());
+            if (s != FBF_OK) return s;
         }
-        const uint64_t st0 = 0, cnt = npoints;
-        fbf_status s = sym_topk(dev, npoints, &st0, &cnt, queries_in, 1, (uint32_t)k, pos.data(), dist.data());
-        if (s != FBF_OK) return s;
     }
     for (size_t j = 0; j < k; ++j) {
         closest_points[j] = pos[j];

@@ -247,6 +247,31 @@ struct SymOneArgs {
     uint32_t *out;         // [2K]
 };
 hipError_t launch_sym_one(const SymOneArgs &a, int B, hipStream_t st);
+// Many independent one-slice searches in one launch (concurrent per-call
+// searches combined): request r scans its slice with nblk workgroups, each
+// into its own list (slots list0 .. list0 + nblk - 1), sharing their
+// thresholds through gtau[r]; the last of them to finish merges the lists
+// into out[r][2K].
+struct SymReq {
+    const int8_t *codes;   // the slice's first row (112-byte rows)
+    uint32_t n;            // rows in the slice
+    uint32_t chunk;        // rows per workgroup (a multiple of kRowTile)
+    uint32_t nblk;         // workgroups: ceil(n / chunk)
+    uint32_t list0;        // the request's first list slot
+    int4 q[7];             // the query's 112 code bytes
+};
+struct SymMultiArgs {
+    const SymReq *reqs;    // [nreq] (device memory)
+    uint32_t nreq;
+    uint32_t K;
+    uint64_t *lists;       // [sum of nblk][B]
+    uint32_t *counts;      // [sum of nblk]
+    uint32_t *done;        // [nreq] arrival counters (zero between calls)
+    uint64_t *gtau;        // [nreq] shared thresholds (all ones between calls)
+    uint32_t *out;         // [nreq][2K]: positions in the slice, then int distances
+};
+// grid_x: the largest nblk of the requests.
+hipError_t launch_sym_multi(const SymMultiArgs &a, uint32_t grid_x, int B, hipStream_t st);
 // variant: 0 BF16, 1 FP8, 2 SQ8, 3 symmetric int8; B: list capacity (128,
 // 256, 512, 1024; at most a.list_stride); nw: waves per workgroup (1, 2, 4),
 // an item carries at most 64 * nw queries.  launch_build_mfma with

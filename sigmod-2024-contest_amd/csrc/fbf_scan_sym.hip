@@ -218,6 +218,121 @@ __global__ __launch_bounds__(64) void sym_one_kernel(SymOneArgs a) {
     if (lane == 0) __hip_atomic_store(a.done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// ---------------------------------------------------------------- many one-slice searches, one launch
+
+template <int B>
+__global__ __launch_bounds__(64) void sym_multi_kernel(SymMultiArgs a) {
+    __shared__ uint64_t sbuf[B];
+    __shared__ int4 stile[kRowTile * kCodeInt4];
+    __shared__ uint32_t s_last;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t rq = blockIdx.y;
+    cptr<SymReq> R = as_const(a.reqs) + rq;
+    const uint32_t nblk = R->nblk;
+    if (blockIdx.x >= nblk) return;  // the request needs fewer workgroups
+    const uint32_t n = R->n, chunk = R->chunk, list0 = R->list0;
+    int4 q[kCodeInt4];
+#pragma unroll
+    for (int c = 0; c < kCodeInt4; ++c) q[c] = R->q[c];
+    const int qq = dot_codes(q, q);
+    const uint32_t r0 = blockIdx.x * chunk;
+    const uint32_t r1 = min(r0 + chunk, n);
+    uint64_t *list = a.lists + (uint64_t)(list0 + blockIdx.x) * B;
+    uint32_t cnt = 0;
+    uint64_t tau = kKeyMax;
+    const int4 *src = reinterpret_cast<const int4 *>(R->codes);
+    uint64_t *gt = a.gtau + rq;
+    uint32_t step = 0;
+    // The next tile is loaded into registers while this one is scored.
+    int4 nxt[kCodeInt4];
+    auto load_tile = [&](uint32_t t) __attribute__((always_inline)) {
+#pragma unroll
+        for (int c = 0; c < kCodeInt4; ++c) {
+            const uint32_t e = lane + 64 * c;
+            const uint32_t row = t + e / kCodeInt4;
+            nxt[c] = row < r1 ? src[(uint64_t)t * kCodeInt4 + e] : make_int4(0, 0, 0, 0);
+        }
+    };
+    load_tile(r0);
+#pragma unroll 1
+    for (uint32_t t = r0; t < r1; t += kRowTile) {
+#pragma unroll
+        for (int c = 0; c < kCodeInt4; ++c) stile[lane + 64 * c] = nxt[c];
+        if (t + kRowTile < r1) load_tile(t + kRowTile);
+        __syncthreads();
+        int4 x[kCodeInt4];
+#pragma unroll
+        for (int c = 0; c < kCodeInt4; ++c) x[c] = stile[lane * kCodeInt4 + c];
+        __syncthreads();
+        const uint32_t row = t + lane;
+        const int dist = dot_codes(x, x) - 2 * dot_codes(x, q) + qq;
+        const uint64_t key = ((uint64_t)(uint32_t)dist << 32) | row;
+        offer_global<B>(list, cnt, tau, a.K, sbuf, row < r1 && key < tau, key);
+        // Every 4 tiles: publish this list's threshold and take the tighter
+        // of it and the request's (the K-th key of any of the request's lists
+        // bounds its K-th key).
+        if ((++step & 3u) == 0) {
+            uint64_t g = 0;
+            if (lane == 0) g = atomicMin((unsigned long long *)gt, (unsigned long long)tau);
+            g = uniform_u64(__shfl(g, 0));
+            tau = g < tau ? g : tau;
+        }
+    }
+    if (lane == 0) {
+        (void)atomicMin((unsigned long long *)gt, (unsigned long long)tau);
+        a.counts[list0 + blockIdx.x] = cnt;
+    }
+    // Publish this list, then count the request's arrivals; its last
+    // workgroup merges.
+    __threadfence();
+    if (lane == 0) {
+        const uint32_t old = __hip_atomic_fetch_add(&a.done[rq], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = old == nblk - 1 ? 1u : 0u;
+    }
+    __syncthreads();
+    if (s_last == 0) return;
+    __threadfence();
+    uint32_t mc = 0;
+    uint64_t mt = kKeyMax;
+#pragma unroll 1
+    for (uint32_t g = 0; g < nblk; ++g) {
+        const uint32_t c = __hip_atomic_load(&a.counts[list0 + g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t *L = a.lists + (uint64_t)(list0 + g) * B;
+#pragma unroll 1
+        for (uint32_t i = 0; i < c; i += 64) {
+            const uint32_t ii = i + lane;
+            const uint64_t key = ii < c ? __hip_atomic_load(&L[ii], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                        : kKeyMax;
+            offer_lds<B>(sbuf, mc, mt, a.K, key < mt, key);
+        }
+    }
+    compact_lds<B>(sbuf, mc, mt, a.K);  // ascending, mc <= K
+    uint32_t *out = a.out + (uint64_t)rq * 2 * a.K;
+    for (uint32_t i = lane; i < a.K; i += 64) {
+        const bool v = i < mc;
+        out[i] = v ? (uint32_t)sbuf[i] : kPadId;
+        out[a.K + i] = v ? (uint32_t)(sbuf[i] >> 32) : 0x7FFFFFFFu;
+    }
+    if (lane == 0) {
+        __hip_atomic_store(gt, kKeyMax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&a.done[rq], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+hipError_t launch_sym_multi(const SymMultiArgs &a, uint32_t grid_x, int B, hipStream_t st) {
+    if (a.nreq == 0) return hipSuccess;
+    if (grid_x == 0 || a.nreq > 65535) return hipErrorInvalidValue;
+    if ((int)a.K + kRowTile > B) return hipErrorInvalidValue;
+    const dim3 grid(grid_x, a.nreq);
+    switch (B) {
+    case 256: hipLaunchKernelGGL(sym_multi_kernel<256>, grid, dim3(64), 0, st, a); break;
+    case 512: hipLaunchKernelGGL(sym_multi_kernel<512>, grid, dim3(64), 0, st, a); break;
+    case 1024: hipLaunchKernelGGL(sym_multi_kernel<1024>, grid, dim3(64), 0, st, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_sym_one(const SymOneArgs &a, int B, hipStream_t st) {
     if (a.n == 0 || a.chunk == 0 || a.chunk % kRowTile != 0) return hipErrorInvalidValue;
     if ((int)a.K + kRowTile > B) return hipErrorInvalidValue;
