@@ -122,3 +122,47 @@ def test_bruteforce_many_threads(pkg, orc, ref_bf):
         s, c = int(starts[i]), int(counts[i])
         sid, _ = orc.bf_sorted(codes[s:s + c], qc[i], 140)
         assert np.array_equal(got[i], sid), i
+
+
+def test_bruteforce_combined_mixed(pkg, orc, ref_bf):
+    """Concurrent calls are combined into shared launches: calls on two
+    registered arrays, with different k (1 to 300, each answered from the
+    batch's largest-k list), slices shorter than k, and bruteforce_subgraph
+    calls among them, from 16 threads -- every result equals the oracle's."""
+    from concurrent.futures import ThreadPoolExecutor
+    fbf = pkg.fbf
+    codes_a = np.ascontiguousarray(ref_bf["codes"])
+    rng = np.random.default_rng(17)
+    codes_b = rng.integers(-127, 128, (30_000, 112), dtype=np.int8)
+    alpha = float(ref_bf["alpha"])
+    n_calls = 400
+    which = rng.integers(0, 2, n_calls)
+    ks = rng.choice([1, 7, 100, 140, 300], n_calls)
+    qc = orc.sq8sym_encode(rng.standard_normal((n_calls, 100)).astype(np.float32), alpha)
+    arrs = (codes_a, codes_b)
+    starts = np.array([rng.integers(0, arrs[w].shape[0] - 1) for w in which])
+    counts = np.array([min(int(rng.integers(1, 6000)), arrs[w].shape[0] - int(s)) for w, s in zip(which, starts)])
+    counts[::17] = np.minimum(counts[::17], 5)  # shorter than k
+    sub = rng.random(n_calls) < 0.3
+    _say(f"case: {n_calls} combined calls from 16 threads, 2 registered arrays, k in 1..300")
+    fbf.register_codes(codes_a)
+    fbf.register_codes(codes_b)
+    try:
+        def one(i):
+            arr = arrs[which[i]]
+            if sub[i]:
+                return fbf.bruteforce_subgraph(int(ks[i]), int(counts[i]), arr, alpha, qc[i], offset_rows=int(starts[i]))
+            return fbf.bruteforce(int(ks[i]), int(counts[i]), arr, alpha, qc[i], offset_rows=int(starts[i])), None
+        with ThreadPoolExecutor(16) as ex:
+            got = list(ex.map(one, range(n_calls)))
+    finally:
+        fbf.unregister_codes(codes_a)
+        fbf.unregister_codes(codes_b)
+    for i in range(n_calls):
+        s, c, k = int(starts[i]), int(counts[i]), int(ks[i])
+        sid, sd = orc.bf_sorted(arrs[which[i]][s:s + c], qc[i], k)
+        pos, dist = got[i]
+        assert np.array_equal(pos, sid), (i, k, c)
+        if dist is not None:
+            valid = sid >= 0
+            assert np.array_equal(dist[valid], sd[valid].astype(np.float32) * np.float32(alpha) * np.float32(alpha)), i
