@@ -61,6 +61,9 @@ def parse():
     p.add_argument("--dropin", action="store_true",
                    help="only time the reference-shaped bruteforce() drop-in on the router-routed slices "
                         "(selectivity below bf_thr / bf_thr3) beside the CPU bruteforce(), and exit")
+    p.add_argument("--subinterval", action="store_true",
+                   help="only time the type-2 sub-interval path (fbf_search_subinterval) on the type-2 queries "
+                        "the router sends to the graph side, beside the oracle's restatement on the host, and exit")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                    help="only run a plain device copy of MIB MiB (HBM counter calibration) and exit")
     return p.parse_args()
@@ -99,6 +102,8 @@ def main() -> int:
     log(f"data ready in {time.time() - t0:.1f}s")
     if a.dropin:
         return dropin(a, ge, fbf, cfg, vecs, labels, ts, q, metas)
+    if a.subinterval:
+        return subinterval_bench(a, ge, fbf, cfg, vecs, labels, ts, q, metas)
 
     mode = a.mode
     repr_flags = fbf.MODE_REPR[fbf.MODES[mode]]
@@ -383,6 +388,51 @@ def main() -> int:
         "cpu_baseline_router_bf": cpu_router,
         "cpu_baseline_reference_router_bf": cpu_ref,
     }
+    print(json.dumps(line), flush=True)
+    ds.close()
+    return 0
+
+
+def subinterval_bench(a, ge, fbf, cfg, vecs, labels, ts, q, metas) -> int:
+    """The router's type-2 path (hybrid_graph.cpp:317-494) for the type-2
+    queries it sends to the graph side (selectivity >= bf_thr = 0.045):
+    SplitInterval over the ten time buckets, the SMALL buckets' int8 scans with
+    bucket-encoded queries, the candidate pool and the global FP32 refine
+    (fbf_search_subinterval; the FULL and MEDIUM buckets' graph searches are
+    the caller's).  Times the whole set on the GPU (1 warmup, then a.steps
+    timed searches, queries resident in HBM) and, as the CPU baseline, the
+    oracle's restatement (orc_subinterval, all host threads) on a sample whose
+    ids it compares."""
+    import torch
+    ds = fbf.Dataset(vecs, labels, ts, repr_flags=fbf.REPR_F32 | fbf.REPR_TBUCKETS)
+    md = torch.from_numpy(np.ascontiguousarray(metas)).cuda()
+    ranges, _ = ds.route(md, order=False)
+    torch.cuda.synchronize()
+    r = ranges.cpu().numpy().view(np.uint32)
+    idx = np.nonzero((metas[:, 0] == 2) & (r[:, 3] == 0))[0]
+    m = np.ascontiguousarray(metas[idx])
+    qd = torch.from_numpy(np.ascontiguousarray(q[idx])).cuda()
+    log(f"sub-interval: {idx.size} graph-routed type-2 queries")
+    ids, _, types = ds.search_subinterval(qd, m, k=a.k)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        ids, _, types = ds.search_subinterval(qd, m, k=a.k)
+    torch.cuda.synchronize()
+    gpu_qps = idx.size * a.steps / (time.perf_counter() - t0)
+    ids = ids.cpu().numpy().view(np.uint32)
+    orc = ge.load_oracle()
+    lay = orc.Layouts(labels.astype(np.int32), ts)
+    tb = orc.TimeBuckets(vecs, lay)
+    s = np.random.default_rng(3).choice(idx.size, min(512, idx.size), replace=False)
+    t0 = time.perf_counter()
+    oid, _, _ = orc.subinterval(vecs, lay, tb, q[idx[s]], m[s], a.k)
+    cpu_qps = s.size / (time.perf_counter() - t0)
+    same = int((ids[s] == oid).all(1).sum())
+    line = {"subinterval": cfg.name, "queries": int(idx.size), "steps": a.steps, "gpu_qps": gpu_qps,
+            "small_slices_per_query": float((types == 0).sum(1).mean()), "cpu_port_qps": cpu_qps,
+            "cpu_sample": int(s.size), "ids_equal_on_sample": f"{same}/{s.size}", **host_cpu()}
+    log(f"sub-interval: GPU {gpu_qps:.0f} QPS, CPU port {cpu_qps:.1f} QPS on {s.size} queries, ids equal {same}/{s.size}")
     print(json.dumps(line), flush=True)
     ds.close()
     return 0
