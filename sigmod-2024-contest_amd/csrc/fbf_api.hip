@@ -864,7 +864,7 @@ static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, 
     // prefix lists start empty, so its waves compact often, and every
     // compaction holds up the other waves that share the tile ring.
     const uint32_t gmax_pre = std::min<uint32_t>(gmax, 128u);
-    // Waves per MFMA work item: each wave holds kMfmaWaveQueries queries.
+    // MFMA item width in units of kMfmaWaveQueries queries.
     auto waves_of = [&](uint32_t g) {
         return !mfma ? 1 : g <= kMfmaWaveQueries ? 1 : g <= 2 * kMfmaWaveQueries ? 2 : 4;
     };
@@ -1071,8 +1071,8 @@ static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, 
         a.counts = S.d_counts.as<uint32_t>();
         a.gtau = gtau;
         a.list_stride = (uint32_t)B;
-        // Waves per workgroup: enough for the largest item of the launch (a
-        // wave with no queries of an item only helps load its rows).
+        // Item width: enough for the largest item of the launch (a wave with
+        // no queries of an item only helps load its rows).
         auto scan = [&](uint64_t i0, uint64_t i1, uint32_t kk, const uint32_t *mask) -> fbf_status {
             const bool pre = seeded && i1 <= n_items_a;
             const int nw = waves_of(pre ? gmax_pre : gmax);

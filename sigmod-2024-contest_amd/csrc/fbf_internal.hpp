@@ -49,7 +49,9 @@ __device__ __host__ inline uint32_t list_count_c1(uint32_t w) { return (w >> 11)
 __device__ __host__ inline uint32_t list_count_h(uint32_t w) { return (w >> 22) << 6; }
 
 constexpr int kMaxGroup = 16;    // queries per work item
-// Queries one wave of the MFMA scans holds: 32-query blocks, two or four.
+// Unit of MFMA item width: an item of width nw (1, 2, 4) carries at most
+// 64 nw queries (widths 1 and 2: one wave per 64 queries; width 4: two waves
+// of 128).
 constexpr uint32_t kMfmaWaveQueries = 64;
 
 // (distance, id) key: distance bits in the high word.  For the non-negative
@@ -273,8 +275,8 @@ struct SymMultiArgs {
 // grid_x: the largest nblk of the requests.
 hipError_t launch_sym_multi(const SymMultiArgs &a, uint32_t grid_x, int B, hipStream_t st);
 // variant: 0 BF16, 1 FP8, 2 SQ8, 3 symmetric int8; B: list capacity (128,
-// 256, 512, 1024; at most a.list_stride); nw: waves per workgroup (1, 2, 4),
-// an item carries at most 64 * nw queries.  launch_build_mfma with
+// 256, 512, 1024; at most a.list_stride); nw: item width (1, 2, 4), an
+// item carries at most 64 * nw queries.  launch_build_mfma with
 // variant 3 reads the layout's symmetric codes from sq8_codes and writes the
 // code norms to norms.
 hipError_t launch_scan_mfma(int variant, const ScanMfmaArgs &a, int B, int nw, hipStream_t st);

@@ -605,9 +605,13 @@ __device__ inline float tau_as_float(uint64_t tau) {
 
 // One workgroup of NW waves per work item.  The item's rows stream through a
 // ring of LDS tiles (loaded once, read by every wave); wave w owns the item's
-// queries [64 w, 64 w + 64) as two 32-query B blocks in VGPRs, its own
-// candidate lists and thresholds, and synchronises with the other waves only
-// at the one barrier per pair of tiles that hands the next pair over.
+// queries [32 QB w, 32 QB (w + 1)) as QB (two or four) 32-query B blocks in
+// VGPRs, its own candidate lists and thresholds, and synchronises with the
+// other waves only at the one barrier per pair of tiles that hands the next
+// pair over.  Four blocks per wave read each LDS fragment for four MFMA
+// chains instead of two, and put 256 queries on two waves instead of four (two
+// waves meet at each barrier); they take 256 VGPRs, so the prefix pass, whose
+// open thresholds make it append and compact far more, keeps two.
 //
 // Candidate lists.  In the MFMA output tile a query's 32 rows sit in two lanes
 // (l and l + 32, 16 rows each), so each query's list of B keys is split in two
@@ -617,9 +621,6 @@ __device__ inline float tau_as_float(uint64_t tau) {
 // cut to the K smallest keys, split evenly between the halves; the threshold
 // becomes the K-th key.  At the end of the item each list is made contiguous
 // (its second half moved up, or the whole list cut to K).
-constexpr int kQB = (int)(kMfmaWaveQueries / 32);  // 32-query blocks per wave (2 or 4)
-constexpr uint32_t kWaveQ = kMfmaWaveQueries;          // queries per wave
-static_assert(kQB == 2 || kQB == 4, "two or four query blocks per wave");
 constexpr uint32_t kShareTiles = 512;  // tiles between threshold exchanges without a compaction
 
 // Bit j (score j of the lane: row r0 + 8 (j / 4) + (j % 4), r0 = tile row +
@@ -695,10 +696,13 @@ __device__ inline void compact_split(uint64_t *list, uint32_t &c0, uint32_t &c1,
     WaveSync::sync();
 }
 
-template <int VAR, int NW, int B>
+template <int VAR, int NW, int B, int QB>
 __device__ __attribute__((always_inline)) void scan_mfma_item(const ScanMfmaArgs &a, const WorkItem &W, uint4 *sT0,
                                                                uint4 *sT1, uint4 *sT2, uint4 *sT3, uint64_t *sbuf,
                                                                uint64_t *s_tau) {
+    static_assert(QB == 2 || QB == 4, "two or four query blocks per wave");
+    constexpr int kQB = QB;                       // 32-query blocks per wave
+    constexpr uint32_t kWaveQ = 32u * QB;         // queries per wave
     constexpr int kU4 = TileBytes<VAR>::value / 16;
     constexpr uint32_t H = B / 2;                          // keys per list half
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -719,8 +723,10 @@ __device__ __attribute__((always_inline)) void scan_mfma_item(const ScanMfmaArgs
     uint32_t qs[kQB], qe[kQB], qib[kQB];
     bool qv[kQB];
     uint32_t cnt[kQB];          // keys in this lane's half of its query's list
-    uint64_t *mylist[kQB];      // this lane's half
-    uint64_t myslot[kQB];
+    // List slot of the lane's query (fewer than 2^32 slots: each takes at
+    // least 1 KiB of device memory); the lane's half starts at key
+    // myslot * list_stride + half * H.
+    uint32_t myslot[kQB];
 #pragma unroll
     for (int b = 0; b < kQB; ++b) {
         const uint32_t j = 32 * b + col;
@@ -733,12 +739,12 @@ __device__ __attribute__((always_inline)) void scan_mfma_item(const ScanMfmaArgs
             qib[b] = e.qi;
             qs[b] = e.s;
             qe[b] = e.e;
-            myslot[b] = list_slot(e, W);
+            myslot[b] = (uint32_t)list_slot(e, W);
             if (a.qmask && a.qmask[e.qi] == 0) qv[b] = false;
         }
         cnt[b] = 0;
-        mylist[b] = a.lists + myslot[b] * (uint64_t)a.list_stride + half * H;
     }
+    auto half_list = [&](int b) { return a.lists + (uint64_t)myslot[b] * a.list_stride + half * H; };
     // The queries j = lane (and j = lane + 64 with four blocks): every
     // per-query loop below runs over exactly those j.
     const uint32_t my_qi0 = half ? qib[1] : qib[0];
@@ -746,9 +752,7 @@ __device__ __attribute__((always_inline)) void scan_mfma_item(const ScanMfmaArgs
     auto qi_of = [&](uint32_t j) { return j >= 64 ? my_qi1 : my_qi0; };
     // Base of the list of query j = 32 b + c (held by lane c, half 0).
     auto list_base = [&](int b, uint32_t c) {
-        const uint64_t p = reinterpret_cast<uint64_t>(mylist[b]);
-        const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)p, c), hi = __builtin_amdgcn_readlane((uint32_t)(p >> 32), c);
-        return reinterpret_cast<uint64_t *>(((uint64_t)hi << 32) | lo);
+        return a.lists + (uint64_t)__builtin_amdgcn_readlane(myslot[b], c) * a.list_stride;
     };
 
     // Start from the thresholds earlier items (or the prefix pass) published.
@@ -831,7 +835,7 @@ __device__ __attribute__((always_inline)) void scan_mfma_item(const ScanMfmaArgs
             const bool full = t >= qsb && t + kMTile <= qeb;
             if (__ballot(pm != 0 && !full)) pm &= range_bits(t, half, qsb, qeb);
         }
-        append_bits(pm, sc, t + 4 * half, lay_bit, mylist[b], cnt[b]);
+        append_bits(pm, sc, t + 4 * half, lay_bit, half_list(b), cnt[b]);
         return true;
     };
     // The loop body, once per tile buffer with a literal buffer number.
@@ -986,13 +990,13 @@ __device__ inline uint32_t xcc_id() {
     return __builtin_amdgcn_s_getreg((20) | (0 << 6) | ((4 - 1) << 11)) & 0xFu;
 }
 
-template <int VAR, int NW, int B>
+template <int VAR, int NW, int B, int QB>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2, 8)))
 void scan_mfma_kernel(ScanMfmaArgs a) {
     __shared__ uint4 sT0[64 * TileChunks<VAR>::value], sT1[64 * TileChunks<VAR>::value],
         sT2[64 * TileChunks<VAR>::value], sT3[64 * TileChunks<VAR>::value];
     __shared__ uint64_t sbuf[NW][B];
-    __shared__ uint64_t s_tau[kWaveQ * NW];
+    __shared__ uint64_t s_tau[32 * QB * NW];
     __shared__ uint32_t s_idx;
     uint64_t *my_sbuf = sbuf[threadIdx.x >> 6];
     // One workgroup per item (queue == null: item blockIdx.x), or the
@@ -1032,44 +1036,46 @@ void scan_mfma_kernel(ScanMfmaArgs a) {
         __syncthreads();
         if (idx == kPadId) break;
         const WorkItem W = as_const(a.items)[idx];
-        if (item_active<NW>(a, W)) scan_mfma_item<VAR, NW, B>(a, W, sT0, sT1, sT2, sT3, my_sbuf, s_tau);
+        if (item_active<NW>(a, W)) scan_mfma_item<VAR, NW, B, QB>(a, W, sT0, sT1, sT2, sT3, my_sbuf, s_tau);
         __syncthreads();
     }
 }
 
-template <int VAR, int NW>
+template <int VAR, int NW, int QB>
 static hipError_t launch_mfma_nw(const ScanMfmaArgs &a, int B, uint64_t grid, hipStream_t st) {
     switch (B) {
-    case 128: hipLaunchKernelGGL((scan_mfma_kernel<VAR, NW, 128>), dim3((uint32_t)grid), dim3(64 * NW), 0, st, a); break;
-    case 256: hipLaunchKernelGGL((scan_mfma_kernel<VAR, NW, 256>), dim3((uint32_t)grid), dim3(64 * NW), 0, st, a); break;
-    case 512: hipLaunchKernelGGL((scan_mfma_kernel<VAR, NW, 512>), dim3((uint32_t)grid), dim3(64 * NW), 0, st, a); break;
-    case 1024: hipLaunchKernelGGL((scan_mfma_kernel<VAR, NW, 1024>), dim3((uint32_t)grid), dim3(64 * NW), 0, st, a); break;
+    case 128: hipLaunchKernelGGL((scan_mfma_kernel<VAR, NW, 128, QB>), dim3((uint32_t)grid), dim3(64 * NW), 0, st, a); break;
+    case 256: hipLaunchKernelGGL((scan_mfma_kernel<VAR, NW, 256, QB>), dim3((uint32_t)grid), dim3(64 * NW), 0, st, a); break;
+    case 512: hipLaunchKernelGGL((scan_mfma_kernel<VAR, NW, 512, QB>), dim3((uint32_t)grid), dim3(64 * NW), 0, st, a); break;
+    case 1024: hipLaunchKernelGGL((scan_mfma_kernel<VAR, NW, 1024, QB>), dim3((uint32_t)grid), dim3(64 * NW), 0, st, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
+// Item widths: nw = 1 and 2 run as nw waves of two query blocks; nw = 4 (256
+// queries; prefix items carry at most 128) as two waves of four blocks.
 template <int VAR>
 static hipError_t launch_mfma_var(const ScanMfmaArgs &a, int B, int nw, uint64_t grid, hipStream_t st) {
-    if (nw == 1) return launch_mfma_nw<VAR, 1>(a, B, grid, st);
-    if (nw == 2) return launch_mfma_nw<VAR, 2>(a, B, grid, st);
-    return launch_mfma_nw<VAR, 4>(a, B, grid, st);
+    if (nw == 1) return launch_mfma_nw<VAR, 1, 2>(a, B, grid, st);
+    if (nw == 2) return launch_mfma_nw<VAR, 2, 2>(a, B, grid, st);
+    return launch_mfma_nw<VAR, 2, 4>(a, B, grid, st);
 }
 
-template <int VAR, int NW>
+template <int VAR, int NW, int QB>
 static int occupancy_nw(int B) {
     int n = 0;
     hipError_t e = hipErrorInvalidValue;
-    if (B == 128) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, scan_mfma_kernel<VAR, NW, 128>, 64 * NW, 0);
-    if (B == 256) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, scan_mfma_kernel<VAR, NW, 256>, 64 * NW, 0);
-    if (B == 512) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, scan_mfma_kernel<VAR, NW, 512>, 64 * NW, 0);
-    if (B == 1024) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, scan_mfma_kernel<VAR, NW, 1024>, 64 * NW, 0);
+    if (B == 128) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, scan_mfma_kernel<VAR, NW, 128, QB>, 64 * NW, 0);
+    if (B == 256) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, scan_mfma_kernel<VAR, NW, 256, QB>, 64 * NW, 0);
+    if (B == 512) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, scan_mfma_kernel<VAR, NW, 512, QB>, 64 * NW, 0);
+    if (B == 1024) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, scan_mfma_kernel<VAR, NW, 1024, QB>, 64 * NW, 0);
     return (e == hipSuccess && n > 0) ? n : 8 / NW;
 }
 
 template <int VAR>
 static int occupancy_var(int B, int nw) {
-    return nw == 1 ? occupancy_nw<VAR, 1>(B) : nw == 2 ? occupancy_nw<VAR, 2>(B) : occupancy_nw<VAR, 4>(B);
+    return nw == 1 ? occupancy_nw<VAR, 1, 2>(B) : nw == 2 ? occupancy_nw<VAR, 2, 2>(B) : occupancy_nw<VAR, 2, 4>(B);
 }
 
 // Resident workgroups per CU of a scan instantiation (cached per variant, B, nw).
@@ -1093,7 +1099,7 @@ void warm_scan_mfma(int variant, int B, int nw) {
         (void)persistent_per_cu(variant, B, nw);
 }
 
-// nw: waves per workgroup (1, 2 or 4); an item carries at most 64 * nw queries.
+// nw: item width (1, 2 or 4); an item carries at most 64 * nw queries.
 hipError_t launch_scan_mfma(int variant, const ScanMfmaArgs &a, int B, int nw, hipStream_t st) {
     if (a.n_items == 0) return hipSuccess;
     if (nw != 1 && nw != 2 && nw != 4) return hipErrorInvalidValue;
