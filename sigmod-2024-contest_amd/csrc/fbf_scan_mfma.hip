@@ -1054,11 +1054,25 @@ static hipError_t launch_mfma_nw(const ScanMfmaArgs &a, int B, uint64_t grid, hi
 }
 
 // Item widths: nw = 1 and 2 run as nw waves of two query blocks; nw = 4 (256
-// queries; prefix items carry at most 128) as two waves of four blocks.
+// queries; prefix items carry at most 128) as two waves of four blocks, except
+// BF16 and SQ8 rows with 1024-key lists: there two four-block waves take 46 KB
+// of LDS (16 KB of sort buffers), so three workgroups (six waves) fit a CU,
+// while four waves of two blocks take 62 KB and two workgroups (eight waves)
+// fit.  Measured on C1: certified exact main pass 92.3 ms as four waves, 101.8
+// as two; FP8 (k' = 896, also 1024-key lists) 80.7 ms as two, 92.3 as four.
+template <int VAR>
+constexpr bool four_waves_of_two(int B) {
+    return (VAR == 0 || VAR == 2) && B == 1024;
+}
+
 template <int VAR>
 static hipError_t launch_mfma_var(const ScanMfmaArgs &a, int B, int nw, uint64_t grid, hipStream_t st) {
     if (nw == 1) return launch_mfma_nw<VAR, 1, 2>(a, B, grid, st);
     if (nw == 2) return launch_mfma_nw<VAR, 2, 2>(a, B, grid, st);
+    if (four_waves_of_two<VAR>(B)) {
+        hipLaunchKernelGGL((scan_mfma_kernel<VAR, 4, 1024, 2>), dim3((uint32_t)grid), dim3(256), 0, st, a);
+        return hipGetLastError();
+    }
     return launch_mfma_nw<VAR, 2, 4>(a, B, grid, st);
 }
 
@@ -1075,6 +1089,11 @@ static int occupancy_nw(int B) {
 
 template <int VAR>
 static int occupancy_var(int B, int nw) {
+    if (nw == 4 && four_waves_of_two<VAR>(B)) {
+        int n = 0;
+        const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, scan_mfma_kernel<VAR, 4, 1024, 2>, 256, 0);
+        return (e == hipSuccess && n > 0) ? n : 2;
+    }
     return nw == 1 ? occupancy_nw<VAR, 1, 2>(B) : nw == 2 ? occupancy_nw<VAR, 2, 2>(B) : occupancy_nw<VAR, 2, 4>(B);
 }
 
