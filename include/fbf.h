@@ -75,7 +75,7 @@ typedef struct fbf_dataset fbf_dataset;
  * chunk_rows is the rows per work item (default: from the batch's total span, 16 Ki to 256 Ki
  * rows, about 32 Ki work items per batch); group_queries the
  * queries sharing one streamed tile (<= 16 for the VALU scans, default 16;
- * <= 256 for the MFMA scans, default 128). */
+ * <= 256 for the MFMA scans, default 256). */
 typedef struct fbf_search_opts {
     uint32_t mode;        /* fbf_mode */
     uint32_t kprime;      /* 0 -> 3k/2 (BF16, SQ8), 4k (SQ8SYM), 9k (FP8) or 720 (EXACT_BF16), at most 896 */

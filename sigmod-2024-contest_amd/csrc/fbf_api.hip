@@ -856,10 +856,10 @@ static fbf_status search_impl(fbf_dataset *ds, Scratch &S, bool shared_scratch, 
     }
     // Rows per work item (0: the planner picks it from the batch).
     const uint32_t chunk = o.chunk_rows ? (o.chunk_rows + kRowTile - 1) / kRowTile * kRowTile : 0u;
-    // MFMA items carry up to 256 queries (4 waves x 64), 128 by default (two
-    // waves share each row tile); VALU items up to 16.
+    // MFMA items carry up to 256 queries, 256 by default (two waves of four
+    // 32-query blocks share each row tile); VALU items up to 16.
     const uint32_t gcap = mfma ? 256u : (uint32_t)kMaxGroup;
-    const uint32_t gmax = o.group_queries ? std::min<uint32_t>(o.group_queries, gcap) : mfma ? 128u : gcap;
+    const uint32_t gmax = o.group_queries ? std::min<uint32_t>(o.group_queries, gcap) : mfma ? 256u : gcap;
     // Prefix-pass items carry at most 128 queries (two waves per tile): the
     // prefix lists start empty, so its waves compact often, and every
     // compaction holds up the other waves that share the tile ring.
