@@ -50,9 +50,11 @@ def test_seeded_equals_unseeded(pkg, big, mode):
     _say(f"case: N=200000 nq=160 mode={mode} unseeded reference")
     ref_i, ref_d = _search(ds, q, metas, k=100, mode=mode, prepass_div=1)
     assert ds.last_rescans() == 0
-    # group 256: the prefix pass runs 2-wave items, the main pass 4-wave items
+    # group 0 (default) and 256: main-pass items run as two waves of four
+    # 32-query blocks, prefix items as two waves of two; group 128: both
+    # passes two waves of two blocks
     for div, margin, persistent, group in [(0, 0, 0, 0), (8, 10, 0, 0), (8, 10, 2, 0), (4, 400, 1, 0), (8, 0, 2, 0),
-                                           (0, 0, 0, 256), (8, 10, 0, 256)]:
+                                           (0, 0, 0, 256), (8, 10, 0, 256), (0, 0, 0, 128), (8, 10, 0, 128)]:
         _say(f"case: mode={mode} prepass_div={div} margin={margin} persistent={persistent} group={group}")
         ids, d = _search(ds, q, metas, k=100, mode=mode, prepass_div=div, prepass_margin=margin,
                          persistent=persistent, group_queries=group)
