@@ -47,15 +47,6 @@ struct WaveSync {
     }
 };
 
-// Keep the compiler from moving LDS accesses across this point.  The LDS
-// operations of one wave are performed in issue order, so this suffices for a
-// wave to read back LDS words it has just written or updated atomically.
-__device__ inline void lds_order() {
-    __asm__ volatile("" ::: "memory");
-    __builtin_amdgcn_wave_barrier();
-    __asm__ volatile("" ::: "memory");
-}
-
 // Ascending bitonic sort of B keys held in LDS by one wave.
 template <int B, class Sync = BlockSync>
 __device__ inline void wave_sort_lds(uint64_t *s) {
