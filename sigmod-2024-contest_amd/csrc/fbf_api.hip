@@ -2340,6 +2340,8 @@ static fbf_status bf_run_batch_(std::vector<BfReq *> &batch) {
         uint64_t chunk = std::max<uint64_t>(2048, (r.n + 63) / 64);
         chunk = (chunk + kRowTile - 1) / kRowTile * kRowTile;
         const uint32_t nblk = (uint32_t)((r.n + chunk - 1) / chunk);
+        // The merging workgroup reads every list's count in one load, one lane each.
+        if (nblk > 64) return fail(FBF_ERR_UNSUPPORTED, "combined call split into more than 64 workgroups");
         SymReq q{};
         q.codes = r.arr + r.start * 112;
         q.n = (uint32_t)r.n;

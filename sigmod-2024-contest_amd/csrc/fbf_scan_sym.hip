@@ -294,17 +294,35 @@ __global__ __launch_bounds__(64) void sym_multi_kernel(SymMultiArgs a) {
     __threadfence();
     uint32_t mc = 0;
     uint64_t mt = kKeyMax;
+    // Every list's count in one load (lane g: list g; nblk <= 64), then each
+    // list's keys with all of its loads in flight at once, issued while the
+    // previous list is offered: about one memory round trip per list rather
+    // than one per 64 keys (C1-like router slices from 16 threads: 26.1 k ->
+    // 29.1 k calls/s).
+    const uint32_t my_cnt = lane < nblk
+        ? __hip_atomic_load(&a.counts[list0 + lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+    constexpr int kChunks = (B - 64) / 64;  // a list holds at most B - 64 keys
+    uint64_t nx[kChunks];
+    auto load_list = [&](uint32_t g) __attribute__((always_inline)) {
+        const uint32_t c = __builtin_amdgcn_readlane(my_cnt, g);
+        const uint64_t *L = a.lists + (uint64_t)(list0 + g) * B;
+#pragma unroll
+        for (int j = 0; j < kChunks; ++j) {
+            const uint32_t ii = 64 * j + lane;
+            nx[j] = ii < c ? __hip_atomic_load(&L[ii], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : kKeyMax;
+        }
+    };
+    load_list(0);
 #pragma unroll 1
     for (uint32_t g = 0; g < nblk; ++g) {
-        const uint32_t c = __hip_atomic_load(&a.counts[list0 + g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint64_t *L = a.lists + (uint64_t)(list0 + g) * B;
-#pragma unroll 1
-        for (uint32_t i = 0; i < c; i += 64) {
-            const uint32_t ii = i + lane;
-            const uint64_t key = ii < c ? __hip_atomic_load(&L[ii], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                                        : kKeyMax;
-            offer_lds<B>(sbuf, mc, mt, a.K, key < mt, key);
-        }
+        const uint32_t c = __builtin_amdgcn_readlane(my_cnt, g);
+        uint64_t kv[kChunks];
+#pragma unroll
+        for (int j = 0; j < kChunks; ++j) kv[j] = nx[j];
+        if (g + 1 < nblk) load_list(g + 1);  // in flight while list g is offered
+#pragma unroll
+        for (int j = 0; j < kChunks; ++j)
+            if ((uint32_t)(64 * j) < c) offer_lds<B>(sbuf, mc, mt, a.K, kv[j] < mt, kv[j]);
     }
     compact_lds<B>(sbuf, mc, mt, a.K);  // ascending, mc <= K
     uint32_t *out = a.out + (uint64_t)rq * 2 * a.K;
