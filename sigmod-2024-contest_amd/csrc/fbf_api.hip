@@ -2302,7 +2302,9 @@ struct BfReq {
     std::string err;
     bool done = false;
 };
-constexpr int kBfLeaders = 2;       // batches in flight per device
+// Batches in flight per device: 3 (C1-like router slices from 16 C++ threads:
+// 1 -> 24.1 k calls/s, 2 -> 29.7 k, 3 -> 31.2 k, 4 -> 30.8 k).
+constexpr int kBfLeaders = 3;
 constexpr size_t kBfMaxBatch = 512;  // requests per batch
 struct BfCombiner {
     std::mutex mu;
